@@ -1,0 +1,133 @@
+/*
+ * mflbm.h -- C ABI of the MI355X (gfx950) colour-gradient lattice Boltzmann time step.
+ *
+ * Drop-in replacement for the GPU boundary of MF-LBM-CUDA.  The reference exposes that
+ * boundary as process globals plus void functions; here one opaque handle owns all device
+ * state and every entry point returns a status (0 = ok, < 0 = error, message via
+ * mflbm_last_error).  Nothing here exits the process.
+ *
+ * Array layouts on upload/download are the reference's (includes/Idx_cpu.h:49-66):
+ * x fastest, then y, z; 1-based interior coordinates with ghost layers:
+ *   s1 = (nx+2)(ny+2)(nz+2)   one ghost layer   (curv, rho/u/v/w)
+ *   s2 = (nx+4)(ny+4)(nz+4)   two ghost layers  (cn_x/y/z, c_norm, walls)
+ *   s4 = (nx+8)(ny+8)(nz+8)   four ghost layers (phi, walls_type, s_n*)
+ *   pdf = 38 * s1             [x, y, z, direction e (19), colour g (2)]
+ * Element type is double for MFLBM_FP64 handles and float for MFLBM_FP32 handles.
+ */
+#ifndef MFLBM_H
+#define MFLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mflbm_handle mflbm_handle;
+
+enum mflbm_precision { MFLBM_FP64 = 0, MFLBM_FP32 = 1 };
+
+/* Step constants.  Field meaning follows the reference globals copied to the device by
+ * copyConstantData (src/main_iteration_GPU.cu:14-47) and the host flags read by
+ * main_iteration_kernel_GPU (src/main_iteration_GPU.cu:1890-1956).  Derived values
+ * (la_nui*, cos_theta, phi_inlet, rho_in/out) are passed already derived, as
+ * src/IO_multiphase.cpp:200-206 and src/Init_multiphase.cpp:173-214 derive them. */
+typedef struct mflbm_params {
+    int64_t nx, ny, nz;
+    int32_t iper, jper, kper;
+    int32_t wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max;
+    int32_t inlet_bc, outlet_bc; /* 2 = Zou-He pressure; 1 = velocity/convective (not built) */
+    double lbm_gamma, force_z, la_nui1, la_nui2, lbm_beta, phi_inlet, sa_inject, cos_theta;
+    double rho_in, rho_out, relaxation;
+    int32_t precision;           /* enum mflbm_precision */
+    int32_t reserved;
+} mflbm_params;
+
+/* Fields for upload/download. */
+enum mflbm_field {
+    MFLBM_PDF = 0,     /* 38*s1 */
+    MFLBM_PHI = 1,     /* s4 */
+    MFLBM_CN_X = 2, MFLBM_CN_Y = 3, MFLBM_CN_Z = 4, MFLBM_C_NORM = 5, /* s2 */
+    MFLBM_CURV = 6,    /* s1 */
+    MFLBM_S_NX = 7, MFLBM_S_NY = 8, MFLBM_S_NZ = 9, /* s4 */
+    MFLBM_WALLS = 10,      /* s2, int8 0/1 */
+    MFLBM_WALLS_TYPE = 11  /* s4, int8 -1/0/1/2 */
+};
+
+/* Monitor sums (src/Monitor.cpp:17-80), accumulated in FP64 on the device for both
+ * precisions.  Caller provides arrays of nz doubles; null pointers are skipped. */
+typedef struct mflbm_monitor_out {
+    double *fl1, *fl2, *vol1, *vol2, *mass1, *mass2, *pre; /* per z-slice, length nz */
+    double umax2;                 /* max |u|^2 over fluid nodes */
+    double usq1, usq2;            /* sum |u|^2 where phi > 0.999 / phi < -0.999 */
+    int64_t n_fluid;              /* fluid nodes in the interior */
+} mflbm_monitor_out;
+
+/* Per-kernel device time of a profiled run (milliseconds, summed over steps). */
+typedef struct mflbm_kernel_times {
+    double collide_odd_ms, collide_even_ms, boundary_ms, phi_extrap_ms, normals_ms, normal_extrap_ms, curvature_ms;
+    int32_t n_odd, n_even;
+} mflbm_kernel_times;
+
+/* Replaces initialization_GPU + MemAllocate_geometry_GPU(1) + MemAllocate_multi_GPU(1)
+ * (src/Init_multiphase_GPU.cu:13-116) and the host geometry preparation that feeds them:
+ * set_walls domain walls (src/Misc.cpp:67-91) and geometry_preprocessing_new
+ * (src/Geometry_preprocessing.cpp:29-403) run on the device.  `geometry` is the
+ * nx*ny*nz voxel array of a .dat file (1 solid, 0 pore), x fastest. */
+int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handle** out);
+
+/* Replaces MemAllocate_*_GPU(2). */
+void mflbm_destroy(mflbm_handle* h);
+
+const char* mflbm_last_error(const mflbm_handle* h);
+
+/* Initial order parameter of a new run (src/Init_multiphase.cpp:299-358), options 1-5. */
+int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0);
+
+/* Equilibrium populations from rho,u,v,w (s1 layout, host memory, double; any may be
+ * null = rho 1 / velocity 0) and the current phi (src/Init_multiphase.cpp:389-443).
+ * This is the warm-start hook. */
+int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w);
+
+/* Colour-gradient chain on the current phi: the reference's host color_gradient()
+ * (src/Phase_gradient.cpp:15-291) that precedes the first step (src/main.cpp:92). */
+int mflbm_color_gradient(mflbm_handle* h);
+
+/* One time step: main_iteration_kernel_GPU without its device-to-host copy
+ * (src/main_iteration_GPU.cu:1890-2055).  Parity from ntime (odd = pull/push). */
+int mflbm_step(mflbm_handle* h, int32_t ntime);
+
+/* nsteps consecutive steps ntime0 .. ntime0+nsteps-1, replayed from captured HIP graphs. */
+int mflbm_run(mflbm_handle* h, int32_t ntime0, int32_t nsteps);
+
+/* Same as mflbm_run but with per-kernel device timing (events between launches). */
+int mflbm_profile(mflbm_handle* h, int32_t ntime0, int32_t nsteps, mflbm_kernel_times* out);
+
+/* Host <-> device copies of one field in the reference layout (the reference's
+ * cudaMemcpy of src/main_iteration_GPU.cu:2059-2076 and src/Init_multiphase_GPU.cu).
+ * bytes must equal the field size. */
+int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes);
+int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes);
+
+/* compute_macro_vars (src/Misc.cpp:222-274) on the device: rho,u,v,w (s1, double) of the
+ * current (after-even-step) populations; null outputs are skipped. */
+int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w);
+
+/* Monitor reductions of src/Monitor.cpp:17-80 on the device in FP64. */
+int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out);
+
+/* Waits for all work of the handle. */
+int mflbm_sync(mflbm_handle* h);
+
+/* Node counts: 0 fluid interior, 1 fluid in [-1,n+2], 2 solid-boundary in [-2,n+3]. */
+int64_t mflbm_count(const mflbm_handle* h, int32_t which);
+
+/* Size in elements of a field. */
+int64_t mflbm_field_size(const mflbm_handle* h, int32_t field);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MFLBM_H */

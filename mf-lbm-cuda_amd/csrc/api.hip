@@ -1,0 +1,485 @@
+// C ABI of libmflbm (include/mflbm.h): device memory, geometry set-up, node lists, the
+// step sequence (captured into HIP graphs), monitor reductions and field transfers.
+//
+// Step sequence per time step (src/main_iteration_GPU.cu:1890-2055):
+//   collide (odd/even)  ->  periodic copies  ->  Zou-He in/outlet  ->
+//   phi extrapolation  ->  normals + wetting  ->  normal extrapolation  ->  curvature
+// The reference synchronises the device after every launch; here all launches of a step
+// are stream-ordered and replayed from a captured graph.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mflbm.h"
+#include "geometry.h"
+#include "lattice.h"
+#include "launch.h"
+
+using namespace mflbm;
+
+struct mflbm_handle {
+    mflbm_params p{};
+    Dims d{};
+    int prec = MFLBM_FP64;
+    int open_z = 0;
+    hipStream_t stream = nullptr;
+    // fields (element type double or float)
+    void *pdf = nullptr, *phi = nullptr, *cnx = nullptr, *cny = nullptr, *cnz = nullptr, *cnorm = nullptr,
+         *curv = nullptr, *snx = nullptr, *sny = nullptr, *snz = nullptr;
+    uint8_t* walls = nullptr;
+    int8_t* wtype = nullptr;
+    uint32_t *l_fluid = nullptr, *l_fluid_ext = nullptr, *l_sb = nullptr;
+    int n_fluid = 0, n_fluid_ext = 0, n_sb = 0;
+    // monitor scratch
+    double *mon_part = nullptr, *mon_out = nullptr;
+    int mon_rows = 0, mon_nsplit = 0;
+    // captured step graphs: [0] even step, [1] odd step
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::string err;
+};
+
+namespace {
+
+size_t esize(const mflbm_handle* h) { return h->prec == MFLBM_FP64 ? sizeof(double) : sizeof(float); }
+
+int fail(mflbm_handle* h, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return -1;
+}
+
+#define HIPCHK(h, call)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) return fail((h), "%s failed: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename T>
+Fields<T> fields(const mflbm_handle* h) {
+    Fields<T> F;
+    F.d = h->d;
+    F.pdf = (T*)h->pdf; F.phi = (T*)h->phi;
+    F.cnx = (T*)h->cnx; F.cny = (T*)h->cny; F.cnz = (T*)h->cnz; F.cnorm = (T*)h->cnorm; F.curv = (T*)h->curv;
+    F.snx = (const T*)h->snx; F.sny = (const T*)h->sny; F.snz = (const T*)h->snz;
+    F.walls = h->walls; F.wtype = h->wtype;
+    return F;
+}
+
+template <typename T>
+StepConst<T> consts(const mflbm_handle* h) {
+    const mflbm_params& p = h->p;
+    StepConst<T> c;
+    c.gamma = (T)p.lbm_gamma; c.force_z = (T)p.force_z; c.nui1 = (T)p.la_nui1; c.nui2 = (T)p.la_nui2;
+    c.beta = (T)p.lbm_beta; c.phi_inlet = (T)p.phi_inlet; c.sa_inject = (T)p.sa_inject; c.cos_theta = (T)p.cos_theta;
+    c.rho_in = (T)p.rho_in; c.rho_out = (T)p.rho_out; c.relaxation = (T)p.relaxation;
+    return c;
+}
+
+NodeLists lists(const mflbm_handle* h) {
+    NodeLists L;
+    L.fluid = h->l_fluid; L.n_fluid = h->n_fluid;
+    L.fluid_ext = h->l_fluid_ext; L.n_fluid_ext = h->n_fluid_ext;
+    L.solid_bnd = h->l_sb; L.n_solid_bnd = h->n_sb;
+    return L;
+}
+
+// Individual kernel groups of a step; `ev` (optional) receives an event after each group.
+template <typename T>
+void enqueue_step(mflbm_handle* h, bool odd, hipEvent_t* ev) {
+    const Fields<T> F = fields<T>(h);
+    const StepConst<T> P = consts<T>(h);
+    const NodeLists L = lists(h);
+    hipStream_t s = h->stream;
+    int e = 0;
+    auto mark = [&]() { if (ev) (void)hipEventRecord(ev[e++], s); };
+    Launch<T>::collide(F, P, L, odd, s);
+    mark();
+    if (h->p.kper || h->p.jper) Launch<T>::periodic(F, h->p.kper, h->p.jper, odd, s);
+    if (h->open_z) {
+        const int in = h->p.inlet_bc == 2, out = h->p.outlet_bc == 2;
+        if (in || out) Launch<T>::zou_he(F, P, in, out, odd, s);
+    }
+    mark();
+    Launch<T>::phi_extrap(F, L, s);
+    mark();
+    Launch<T>::normals(F, P, L, s);
+    mark();
+    Launch<T>::normal_extrap(F, L, s);
+    mark();
+    Launch<T>::curvature(F, L, s);
+    mark();
+}
+
+template <typename T>
+void enqueue_gradient(mflbm_handle* h) {
+    const Fields<T> F = fields<T>(h);
+    const StepConst<T> P = consts<T>(h);
+    const NodeLists L = lists(h);
+    Launch<T>::phi_extrap(F, L, h->stream);
+    Launch<T>::normals(F, P, L, h->stream);
+    Launch<T>::normal_extrap(F, L, h->stream);
+    Launch<T>::curvature(F, L, h->stream);
+}
+
+void step_dispatch(mflbm_handle* h, bool odd, hipEvent_t* ev) {
+    if (h->prec == MFLBM_FP64) enqueue_step<double>(h, odd, ev);
+    else enqueue_step<float>(h, odd, ev);
+}
+
+void free_all(mflbm_handle* h) {
+    for (int g = 0; g < 2; ++g)
+        if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
+    void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
+                    h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
+    h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
+    h->mon_part = h->mon_out = nullptr;
+    if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
+}
+
+int field_info(const mflbm_handle* h, int field, int64_t* count, size_t* elem, void** ptr) {
+    const Dims& d = h->d;
+    switch (field) {
+        case MFLBM_PDF: *count = 38 * d.S1; *elem = esize(h); *ptr = h->pdf; return 0;
+        case MFLBM_PHI: *count = d.S4; *elem = esize(h); *ptr = h->phi; return 0;
+        case MFLBM_CN_X: *count = d.S2; *elem = esize(h); *ptr = h->cnx; return 0;
+        case MFLBM_CN_Y: *count = d.S2; *elem = esize(h); *ptr = h->cny; return 0;
+        case MFLBM_CN_Z: *count = d.S2; *elem = esize(h); *ptr = h->cnz; return 0;
+        case MFLBM_C_NORM: *count = d.S2; *elem = esize(h); *ptr = h->cnorm; return 0;
+        case MFLBM_CURV: *count = d.S1; *elem = esize(h); *ptr = h->curv; return 0;
+        case MFLBM_S_NX: *count = d.S4; *elem = esize(h); *ptr = h->snx; return 0;
+        case MFLBM_S_NY: *count = d.S4; *elem = esize(h); *ptr = h->sny; return 0;
+        case MFLBM_S_NZ: *count = d.S4; *elem = esize(h); *ptr = h->snz; return 0;
+        case MFLBM_WALLS: *count = d.S2; *elem = 1; *ptr = h->walls; return 0;
+        case MFLBM_WALLS_TYPE: *count = d.S4; *elem = 1; *ptr = h->wtype; return 0;
+        default: return -1;
+    }
+}
+
+int build_graphs(mflbm_handle* h) {
+    for (int g = 0; g < 2; ++g) {
+        if (h->graph[g]) continue;
+        hipGraph_t graph = nullptr;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        step_dispatch(h, g == 1, nullptr);
+        hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (e != hipSuccess) return fail(h, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&h->graph[g], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(h, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+void invalidate_graphs(mflbm_handle* h) {
+    for (int g = 0; g < 2; ++g)
+        if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mflbm_last_error(const mflbm_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handle** out) {
+    if (!out) return -1;
+    *out = nullptr;
+    mflbm_handle* h = new mflbm_handle();
+    *out = h;
+    if (!params || !geometry) return fail(h, "null params or geometry");
+    h->p = *params;
+    const mflbm_params& p = h->p;
+    if (p.precision != MFLBM_FP64 && p.precision != MFLBM_FP32) return fail(h, "unknown precision %d", p.precision);
+    h->prec = p.precision;
+    if (p.nx < 3 || p.ny < 3 || p.nz < 3) return fail(h, "domain must be at least 3 nodes per axis");
+    if (p.nx + 8 > 1023 || p.ny + 8 > 2047 || p.nz + 8 > 2047)
+        return fail(h, "domain too large for packed node coordinates (nx <= 1015, ny, nz <= 2039)");
+    if (p.iper != 0 || p.wall_x_min != 1 || p.wall_x_max != 1)
+        return fail(h, "x must have walls at both ends and no periodicity (src/IO_multiphase.cpp:210-213)");
+    h->open_z = (p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0);
+    if (h->open_z && (p.inlet_bc == 1 || p.outlet_bc == 1))
+        return fail(h, "velocity inlet / convective outlet boundary conditions are not built");
+    h->d = make_dims((int)p.nx, (int)p.ny, (int)p.nz);
+    const Dims& d = h->d;
+    if ((double)d.S1 * 38.0 > 2.0e9 * 8.0) return fail(h, "domain too large");
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+    const size_t es = esize(h);
+    HIPCHK(h, hipMalloc(&h->pdf, 38 * d.S1 * es));
+    HIPCHK(h, hipMalloc(&h->phi, d.S4 * es));
+    HIPCHK(h, hipMalloc(&h->cnx, d.S2 * es));
+    HIPCHK(h, hipMalloc(&h->cny, d.S2 * es));
+    HIPCHK(h, hipMalloc(&h->cnz, d.S2 * es));
+    HIPCHK(h, hipMalloc(&h->cnorm, d.S2 * es));
+    HIPCHK(h, hipMalloc(&h->curv, d.S1 * es));
+    HIPCHK(h, hipMalloc(&h->snx, d.S4 * es));
+    HIPCHK(h, hipMalloc(&h->sny, d.S4 * es));
+    HIPCHK(h, hipMalloc(&h->snz, d.S4 * es));
+    HIPCHK(h, hipMalloc(&h->walls, d.S2));
+    HIPCHK(h, hipMalloc(&h->wtype, d.S4));
+    HIPCHK(h, hipMemsetAsync(h->pdf, 0, 38 * d.S1 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->phi, 0, d.S4 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cnx, 0, d.S2 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cny, 0, d.S2 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cnz, 0, d.S2 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->cnorm, 0, d.S2 * es, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->curv, 0, d.S1 * es, h->stream));
+
+    // domain walls on the voxel flags (src/Misc.cpp:67-78)
+    const int64_t nx = p.nx, ny = p.ny, nz = p.nz, N0 = nx * ny * nz;
+    std::vector<int8_t> g0(N0);
+    for (int64_t n = 0; n < N0; ++n) g0[n] = geometry[n] ? 1 : 0;
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                const int64_t n = i + nx * (j + ny * k);
+                if ((p.wall_z_min == 1 && k == 0) || (p.wall_z_max == 1 && k == nz - 1) || (p.wall_x_min == 1 && i == 0) ||
+                    (p.wall_x_max == 1 && i == nx - 1) || (p.wall_y_min == 1 && j == 0) || (p.wall_y_max == 1 && j == ny - 1))
+                    g0[n] = 1;
+            }
+    int8_t* geo_dev = nullptr;
+    HIPCHK(h, hipMalloc(&geo_dev, N0));
+    HIPCHK(h, hipMemcpy(geo_dev, g0.data(), N0, hipMemcpyHostToDevice));
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr;
+    if (h->prec == MFLBM_FP64) {
+        sx = (double*)h->snx; sy = (double*)h->sny; sz = (double*)h->snz;
+    } else {
+        HIPCHK(h, hipMalloc(&sx, d.S4 * sizeof(double)));
+        HIPCHK(h, hipMalloc(&sy, d.S4 * sizeof(double)));
+        HIPCHK(h, hipMalloc(&sz, d.S4 * sizeof(double)));
+    }
+    hipError_t ge = geometry_preprocess(d, (int)p.iper, (int)p.jper, (int)p.kper, geo_dev, h->walls, h->wtype, sx, sy, sz,
+                                        h->stream);
+    hipFree(geo_dev);
+    if (ge != hipSuccess) return fail(h, "geometry preprocessing failed: %s", hipGetErrorString(ge));
+    if (h->prec == MFLBM_FP32) {
+        Launch<float>::to_real(sx, (float*)h->snx, d.S4, h->stream);
+        Launch<float>::to_real(sy, (float*)h->sny, d.S4, h->stream);
+        Launch<float>::to_real(sz, (float*)h->snz, d.S4, h->stream);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        hipFree(sx); hipFree(sy); hipFree(sz);
+    }
+
+    // node lists from the classified geometry
+    std::vector<uint8_t> walls(d.S2);
+    std::vector<int8_t> wtype(d.S4);
+    HIPCHK(h, hipMemcpy(walls.data(), h->walls, d.S2, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(wtype.data(), h->wtype, d.S4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> lf, lfe, lsb;
+    for (int k = -2; k <= d.nz + 3; ++k)
+        for (int j = -2; j <= d.ny + 3; ++j)
+            for (int i = -2; i <= d.nx + 3; ++i) {
+                const bool in2 = i >= -1 && j >= -1 && k >= -1 && i <= d.nx + 2 && j <= d.ny + 2 && k <= d.nz + 2;
+                if (in2 && walls[i2(d, i, j, k)] == 0) {
+                    lfe.push_back(pack_ijk(i, j, k));
+                    if (i >= 1 && j >= 1 && k >= 1 && i <= d.nx && j <= d.ny && k <= d.nz) lf.push_back(pack_ijk(i, j, k));
+                }
+                if (wtype[i4(d, i, j, k)] == 2) lsb.push_back(pack_ijk(i, j, k));
+            }
+    h->n_fluid = (int)lf.size(); h->n_fluid_ext = (int)lfe.size(); h->n_sb = (int)lsb.size();
+    auto upload_list = [&](uint32_t** dst, const std::vector<uint32_t>& v) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(uint32_t));
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(*dst, v.data(), v.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    };
+    HIPCHK(h, upload_list(&h->l_fluid, lf));
+    HIPCHK(h, upload_list(&h->l_fluid_ext, lfe));
+    HIPCHK(h, upload_list(&h->l_sb, lsb));
+
+    // monitor scratch: blocks of ~8 rows per slice
+    h->mon_rows = std::max(1, std::min(d.ny, 8));
+    h->mon_nsplit = (d.ny + h->mon_rows - 1) / h->mon_rows;
+    HIPCHK(h, hipMalloc(&h->mon_part, (size_t)d.nz * h->mon_nsplit * 10 * sizeof(double)));
+    HIPCHK(h, hipMalloc(&h->mon_out, (size_t)d.nz * 10 * sizeof(double)));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+void mflbm_destroy(mflbm_handle* h) {
+    if (!h) return;
+    free_all(h);
+    delete h;
+}
+
+int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0) {
+    if (!h) return -1;
+    if (option < 1 || option > 5)
+        return fail(h, "initial_fluid_distribution_option %d not supported (options 1-5)", option);
+    if (h->prec == MFLBM_FP64) Launch<double>::init_phi(fields<double>(h), option, interface_z0, h->p.phi_inlet, h->open_z, h->stream);
+    else Launch<float>::init_phi(fields<float>(h), option, interface_z0, h->p.phi_inlet, h->open_z, h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w) {
+    if (!h) return -1;
+    const int64_t n = h->d.S1;
+    const double* src[4] = {rho, u, v, w};
+    double* dv[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int a = 0; a < 4; ++a)
+        if (src[a]) {
+            HIPCHK(h, hipMalloc(&dv[a], n * sizeof(double)));
+            HIPCHK(h, hipMemcpy(dv[a], src[a], n * sizeof(double), hipMemcpyHostToDevice));
+        }
+    if (h->prec == MFLBM_FP64) Launch<double>::init_eq(fields<double>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    else Launch<float>::init_eq(fields<float>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    hipError_t e = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    for (int a = 0; a < 4; ++a)
+        if (dv[a]) (void)hipFree(dv[a]);
+    if (e != hipSuccess || e2 != hipSuccess) return fail(h, "init_equilibrium failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    return 0;
+}
+
+int mflbm_color_gradient(mflbm_handle* h) {
+    if (!h) return -1;
+    if (h->prec == MFLBM_FP64) enqueue_gradient<double>(h);
+    else enqueue_gradient<float>(h);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mflbm_step(mflbm_handle* h, int32_t ntime) {
+    if (!h) return -1;
+    step_dispatch(h, (ntime % 2) != 0, nullptr);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mflbm_run(mflbm_handle* h, int32_t ntime0, int32_t nsteps) {
+    if (!h) return -1;
+    if (nsteps <= 0) return 0;
+    if (build_graphs(h)) return -1;
+    for (int32_t t = ntime0; t < ntime0 + nsteps; ++t) HIPCHK(h, hipGraphLaunch(h->graph[(t % 2) != 0], h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mflbm_profile(mflbm_handle* h, int32_t ntime0, int32_t nsteps, mflbm_kernel_times* out) {
+    if (!h || !out) return -1;
+    std::memset(out, 0, sizeof(*out));
+    constexpr int NE = 7;
+    hipEvent_t ev[NE];
+    for (int e = 0; e < NE; ++e) HIPCHK(h, hipEventCreate(&ev[e]));
+    for (int32_t t = ntime0; t < ntime0 + nsteps; ++t) {
+        const bool odd = (t % 2) != 0;
+        HIPCHK(h, hipEventRecord(ev[0], h->stream));
+        step_dispatch(h, odd, ev + 1);
+        HIPCHK(h, hipEventSynchronize(ev[NE - 1]));
+        float ms[NE - 1];
+        for (int e = 0; e < NE - 1; ++e) HIPCHK(h, hipEventElapsedTime(&ms[e], ev[e], ev[e + 1]));
+        if (odd) { out->collide_odd_ms += ms[0]; out->n_odd++; }
+        else { out->collide_even_ms += ms[0]; out->n_even++; }
+        out->boundary_ms += ms[1];
+        out->phi_extrap_ms += ms[2];
+        out->normals_ms += ms[3];
+        out->normal_extrap_ms += ms[4];
+        out->curvature_ms += ms[5];
+    }
+    for (int e = 0; e < NE; ++e) (void)hipEventDestroy(ev[e]);
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes) {
+    if (!h || !dst) return -1;
+    int64_t n; size_t es; void* ptr;
+    if (field_info(h, field, &n, &es, &ptr)) return fail(h, "unknown field %d", field);
+    if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(dst, ptr, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) {
+    if (!h || !src) return -1;
+    if (field == MFLBM_WALLS || field == MFLBM_WALLS_TYPE) return fail(h, "geometry fields are derived; recreate the handle");
+    int64_t n; size_t es; void* ptr;
+    if (field_info(h, field, &n, &es, &ptr)) return fail(h, "unknown field %d", field);
+    if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w) {
+    if (!h) return -1;
+    const int64_t n = h->d.S1;
+    double* host[4] = {rho, u, v, w};
+    double* dv[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int a = 0; a < 4; ++a)
+        if (host[a]) {
+            HIPCHK(h, hipMalloc(&dv[a], n * sizeof(double)));
+            HIPCHK(h, hipMemsetAsync(dv[a], 0, n * sizeof(double), h->stream));
+        }
+    if (h->prec == MFLBM_FP64) Launch<double>::macro(fields<double>(h), consts<double>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    else Launch<float>::macro(fields<float>(h), consts<float>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    for (int a = 0; a < 4; ++a)
+        if (dv[a]) {
+            if (e == hipSuccess) e = hipMemcpy(host[a], dv[a], n * sizeof(double), hipMemcpyDeviceToHost);
+            (void)hipFree(dv[a]);
+        }
+    if (e != hipSuccess) return fail(h, "macro failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out) {
+    if (!h || !out) return -1;
+    const int nz = h->d.nz;
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::monitor(fields<double>(h), consts<double>(h), h->mon_rows, h->mon_nsplit, h->mon_part, h->mon_out, h->stream);
+    else
+        Launch<float>::monitor(fields<float>(h), consts<float>(h), h->mon_rows, h->mon_nsplit, h->mon_part, h->mon_out, h->stream);
+    HIPCHK(h, hipGetLastError());
+    std::vector<double> res((size_t)10 * nz);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(res.data(), h->mon_out, res.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double* dst[7] = {out->fl1, out->fl2, out->vol1, out->vol2, out->mass1, out->mass2, out->pre};
+    for (int m = 0; m < 7; ++m)
+        if (dst[m]) std::memcpy(dst[m], res.data() + (size_t)m * nz, nz * sizeof(double));
+    double umax = 0.0, u1 = 0.0, u2 = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        umax = std::max(umax, res[(size_t)7 * nz + k]);
+        u1 += res[(size_t)8 * nz + k];
+        u2 += res[(size_t)9 * nz + k];
+    }
+    out->umax2 = umax; out->usq1 = u1; out->usq2 = u2;
+    out->n_fluid = h->n_fluid;
+    return 0;
+}
+
+int mflbm_sync(mflbm_handle* h) {
+    if (!h) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int64_t mflbm_count(const mflbm_handle* h, int32_t which) {
+    if (!h) return -1;
+    return which == 0 ? h->n_fluid : which == 1 ? h->n_fluid_ext : which == 2 ? h->n_sb : -1;
+}
+
+int64_t mflbm_field_size(const mflbm_handle* h, int32_t field) {
+    if (!h) return -1;
+    int64_t n; size_t es; void* ptr;
+    if (field_info(h, field, &n, &es, &ptr)) return -1;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,728 @@
+// HIP kernels of the colour-gradient step, templated on the storage/arithmetic type T.
+// Included by kernels_f64.hip (T = double, built with -ffp-contract=off so every
+// operation rounds once, in source order, like the FP64 oracle) and kernels_f32.hip
+// (T = float, contraction allowed).
+//
+// Reference correspondence (src/main_iteration_GPU.cu):
+//   k_collide<ODD>     kernel_odd_color_GPU / kernel_even_color_GPU   (56-388 / 395-726)
+//   k_periodic<ODD>    perioidc_BC_{Z,Y,ZY}_{odd,even}, periodic_phi_{Z,Y,ZY} (1529-1733)
+//   k_zou_he<AFTER>    inlet/outlet_Zou_He_pressure_BC_{before,after}_odd_GPU (1085-1524)
+//   k_phi_extrap       extrapolate_phi_toSolid                        (732-755)
+//   k_normals          normalDirectionsOfInterfaces + alter_color_gradient_solid_surface_GPU
+//                      (757-878), fused: the wetting correction only reads the node's own
+//                      freshly computed normal
+//   k_normal_extrap    extrapolateNormalToSolid                       (880-906)
+//   k_curvature        CSF_Forces                                     (908-1003)
+// The per-node arithmetic keeps the reference's expression order; that order is what
+// makes the FP64 results bit-identical to the oracle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include "lattice.h"
+
+namespace mflbm {
+namespace dev {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.  Remap the
+// hardware block index so that each XCD works on one contiguous range of the node list:
+// neighbouring nodes (which share cache lines and stencil data) then meet in the same L2.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+template <typename T>
+struct W {
+    static constexpr T w0 = T(1.0) / T(3.0);
+    static constexpr T w1 = T(1.0) / T(18.0);
+    static constexpr T w2 = T(1.0) / T(36.0);
+};
+
+template <typename T>
+__device__ __forceinline__ T wq(int q) { return q == 0 ? W<T>::w0 : (q <= 6 ? W<T>::w1 : W<T>::w2); }
+
+// ---------------------------------------------------------------------------------------
+// Collision + LKR recolouring of one node (src/main_iteration_GPU.cu:116-345).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
+                                          T cnz, T curv, T cnorm, T (&o1)[19], T (&o2)[19]) {
+    T ft[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) ft[q] = g1[q] + g2[q];
+    T rho1 = g1[0], rho2 = g2[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) { rho1 = rho1 + g1[q]; rho2 = rho2 + g2[q]; }
+    const T phi = (rho1 - rho2) / (rho1 + rho2);
+
+    T tmp = T(0.5) * P.gamma * curv * cnorm;
+    const T fx = tmp * cnx, fy = tmp * cny, fz = tmp * cnz + P.force_z;
+
+    const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
+    const T s_nu = omega, s_e = T(1.19), s_e2 = T(1.4), s_pi = T(1.4), s_q = T(1.2), s_t = T(1.98);
+
+    const T den = rho1 + rho2;
+    const T ux = ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] + T(0.5) * fx;
+    const T uy = ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] + T(0.5) * fy;
+    const T uz = ft[5] - ft[6] + ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18] + T(0.5) * fz;
+    const T u2 = ux * ux + uy * uy + uz * uz;
+
+    T sum1 = ft[1] + ft[2] + ft[3] + ft[4] + ft[5] + ft[6];
+    T sum2 = ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] + ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18];
+    T sum3 = ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14];
+    T sum4 = ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18];
+    T sum5 = ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18];
+    T sum6 = T(2.0) * (ft[1] + ft[2]) - ft[3] - ft[4] - ft[5] - ft[6];
+    T sum7 = ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] + ft[13] + ft[14] - T(2.0) * (ft[15] + ft[16] + ft[17] + ft[18]);
+    T sum8 = ft[3] + ft[4] - ft[5] - ft[6];
+    T sum9 = ft[7] + ft[8] + ft[9] + ft[10] - ft[11] - ft[12] - ft[13] - ft[14];
+
+    T m_rho = den;
+    T m_e = T(-30.0) * ft[0] - T(11.0) * sum1 + T(8.0) * sum2;
+    T m_e2 = T(12.0) * ft[0] - T(4.0) * sum1 + sum2;
+    T m_jx = ft[1] - ft[2] + sum3;
+    T m_qx = T(-4.0) * (ft[1] - ft[2]) + sum3;
+    T m_jy = ft[3] - ft[4] + sum4;
+    T m_qy = T(-4.0) * (ft[3] - ft[4]) + sum4;
+    T m_jz = ft[5] - ft[6] + sum5;
+    T m_qz = T(-4.0) * (ft[5] - ft[6]) + sum5;
+    T m_3pxx = sum6 + sum7;
+    T m_3pixx = T(-2.0) * sum6 + sum7;
+    T m_pww = sum8 + sum9;
+    T m_piww = T(-2.0) * sum8 + sum9;
+    T m_pxy = ft[7] - ft[8] - ft[9] + ft[10];
+    T m_pyz = ft[15] - ft[16] - ft[17] + ft[18];
+    T m_pzx = ft[11] - ft[12] - ft[13] + ft[14];
+    T m_tx = ft[7] - ft[8] + ft[9] - ft[10] - ft[11] + ft[12] - ft[13] + ft[14];
+    T m_ty = -ft[7] - ft[8] + ft[9] + ft[10] + ft[15] - ft[16] + ft[17] - ft[18];
+    T m_tz = ft[11] + ft[12] - ft[13] - ft[14] - ft[15] - ft[16] + ft[17] + ft[18];
+
+    const T c23 = T(-0.666666666666666667), c13 = T(0.333333333333333333);
+    const T e2c1 = T(0.0), e2c2 = T(-475.0) / T(63.0), oxx = T(0.0);
+    m_e = m_e - s_e * (m_e - (T(-11.0) * den + T(19.0) * u2)) + (T(38.0) - T(19.0) * s_e) * (fx * ux + fy * uy + fz * uz);
+    m_e2 = m_e2 - s_e2 * (m_e2 - (e2c1 * den + e2c2 * u2)) + (T(-11.0) + T(5.5) * s_e2) * (fx * ux + fy * uy + fz * uz);
+    m_jx = m_jx + fx;
+    m_qx = m_qx - s_q * (m_qx - (c23 * ux)) + (c23 + c13 * s_q) * fx;
+    m_jy = m_jy + fy;
+    m_qy = m_qy - s_q * (m_qy - (c23 * uy)) + (c23 + c13 * s_q) * fy;
+    m_jz = m_jz + fz;
+    m_qz = m_qz - s_q * (m_qz - (c23 * uz)) + (c23 + c13 * s_q) * fz;
+    m_3pxx = m_3pxx - s_nu * (m_3pxx - (T(3.0) * ux * ux - u2)) + (T(2.0) - s_nu) * (T(2.0) * fx * ux - fy * uy - fz * uz);
+    m_3pixx = m_3pixx - s_pi * (m_3pixx - oxx * (T(3.0) * ux * ux - u2)) + (T(1.0) - T(0.5) * s_pi) * (T(-2.0) * fx * ux + fy * uy + fz * uz);
+    m_pww = m_pww - s_nu * (m_pww - (uy * uy - uz * uz)) + (T(2.0) - s_nu) * (fy * uy - fz * uz);
+    m_piww = m_piww - s_pi * (m_piww - oxx * (uy * uy - uz * uz)) + (T(1.0) - T(0.5) * s_pi) * (-fy * uy + fz * uz);
+    m_pxy = m_pxy - s_nu * (m_pxy - (ux * uy)) + (T(1.0) - T(0.5) * s_nu) * (fx * uy + fy * ux);
+    m_pyz = m_pyz - s_nu * (m_pyz - (uy * uz)) + (T(1.0) - T(0.5) * s_nu) * (fy * uz + fz * uy);
+    m_pzx = m_pzx - s_nu * (m_pzx - (ux * uz)) + (T(1.0) - T(0.5) * s_nu) * (fx * uz + fz * ux);
+    m_tx = m_tx - s_t * (m_tx);
+    m_ty = m_ty - s_t * (m_ty);
+    m_tz = m_tz - s_t * (m_tz);
+
+    const T c72 = T(1.0) / T(72.0);
+    m_rho = (T(1.0) / T(19.0)) * m_rho;
+    m_e = (T(1.0) / T(2394.0)) * m_e;
+    m_e2 = (T(1.0) / T(252.0)) * m_e2;
+    m_jx = T(0.1) * m_jx;  m_qx = T(0.025) * m_qx;
+    m_jy = T(0.1) * m_jy;  m_qy = T(0.025) * m_qy;
+    m_jz = T(0.1) * m_jz;  m_qz = T(0.025) * m_qz;
+    m_3pxx = T(2.0) * c72 * m_3pxx;
+    m_3pixx = c72 * m_3pixx;
+    m_pww = T(6.0) * c72 * m_pww;
+    m_piww = T(3.0) * c72 * m_piww;
+    m_pxy = T(0.25) * m_pxy;  m_pyz = T(0.25) * m_pyz;  m_pzx = T(0.25) * m_pzx;
+    m_tx = T(0.125) * m_tx;   m_ty = T(0.125) * m_ty;   m_tz = T(0.125) * m_tz;
+    sum1 = m_rho - T(11.0) * m_e - T(4.0) * m_e2;
+    sum2 = T(2.0) * m_3pxx - T(4.0) * m_3pixx;
+    sum3 = m_pww - T(2.0) * m_piww;
+    sum4 = m_rho + T(8.0) * m_e + m_e2;
+    sum5 = m_jx + m_qx;
+    sum6 = m_jy + m_qy;
+    sum7 = m_jz + m_qz;
+    sum8 = m_3pxx + m_3pixx;
+    sum9 = m_pww + m_piww;
+    T f[19];
+    f[0] = m_rho - T(30.0) * m_e + T(12.0) * m_e2;
+    f[1] = sum1 + m_jx - T(4.0) * m_qx + sum2;
+    f[2] = sum1 - m_jx + T(4.0) * m_qx + sum2;
+    f[3] = sum1 + m_jy - T(4.0) * m_qy - T(0.5) * sum2 + sum3;
+    f[4] = sum1 - m_jy + T(4.0) * m_qy - T(0.5) * sum2 + sum3;
+    f[5] = sum1 + m_jz - T(4.0) * m_qz - T(0.5) * sum2 - sum3;
+    f[6] = sum1 - m_jz + T(4.0) * m_qz - T(0.5) * sum2 - sum3;
+    f[7] = sum4 + sum5 + sum6 + sum8 + sum9 + m_pxy + m_tx - m_ty;
+    f[8] = sum4 - sum5 + sum6 + sum8 + sum9 - m_pxy - m_tx - m_ty;
+    f[9] = sum4 + sum5 - sum6 + sum8 + sum9 - m_pxy + m_tx + m_ty;
+    f[10] = sum4 - sum5 - sum6 + sum8 + sum9 + m_pxy - m_tx + m_ty;
+    f[11] = sum4 + sum5 + sum7 + sum8 - sum9 + m_pzx - m_tx + m_tz;
+    f[12] = sum4 - sum5 + sum7 + sum8 - sum9 - m_pzx + m_tx + m_tz;
+    f[13] = sum4 + sum5 - sum7 + sum8 - sum9 - m_pzx - m_tx - m_tz;
+    f[14] = sum4 - sum5 - sum7 + sum8 - sum9 + m_pzx + m_tx - m_tz;
+    f[15] = sum4 + sum6 + sum7 - sum8 * T(2.0) + m_pyz + m_ty - m_tz;
+    f[16] = sum4 - sum6 + sum7 - sum8 * T(2.0) - m_pyz - m_ty - m_tz;
+    f[17] = sum4 + sum6 - sum7 - sum8 * T(2.0) - m_pyz + m_ty + m_tz;
+    f[18] = sum4 - sum6 - sum7 - sum8 * T(2.0) + m_pyz - m_ty + m_tz;
+
+    const T tmp1 = rho1 / den;
+    o1[0] = tmp1 * f[0];
+    o2[0] = f[0] * (T(1.0) - tmp1);
+    tmp = rho1 * rho2 * P.beta / den;
+    const T rk2 = T(1.0) / sqrt(T(2.0)) / T(36.0);
+    const T w1 = T(1.0) / T(18.0);
+    const T pr[19] = {T(0), cnx, -cnx, cny, -cny, cnz, -cnz,
+                      cnx + cny, -cnx + cny, cnx - cny, -cnx - cny,
+                      cnx + cnz, -cnx + cnz, cnx - cnz, -cnx - cnz,
+                      cny + cnz, -cny + cnz, cny - cnz, -cny - cnz};
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const T w = q <= 6 ? w1 : rk2;
+        o1[q] = tmp1 * f[q] + w * tmp * (pr[q]);
+        o2[q] = f[q] - o1[q];
+    }
+    return phi;
+}
+
+// ---------------------------------------------------------------------------------------
+// AA-pattern collide/stream over the interior fluid-node list.
+// ODD:  pull q from (x - e_q).slot q, push q to (x + e_q).slot opp(q)
+// EVEN: read q from x.slot opp(q), write q to x.slot q
+// Each (node, slot) is touched by exactly one node per sweep, so the update is in place.
+// ---------------------------------------------------------------------------------------
+template <typename T, bool ODD>
+__global__ __launch_bounds__(256) void k_collide(Fields<T> F, StepConst<T> P, const uint32_t* __restrict__ list,
+                                                 int n) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c = i1(d, i, j, k);
+    const int64_t S1 = d.S1;
+    T g1[19], g2[19], o1[19], o2[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const int src = ODD ? c - (EX[q] + d.X1 * (EY[q] + d.Y1 * EZ[q])) : c;
+        const int slot = ODD ? q : OPP[q];
+        g1[q] = F.pdf[src + S1 * slot];
+        g2[q] = F.pdf[src + S1 * (slot + 19)];
+    }
+    const int c2 = i2(d, i, j, k);
+    const T ph = collide_node<T>(P, g1, g2, F.cnx[c2], F.cny[c2], F.cnz[c2], F.curv[c], F.cnorm[c2], o1, o2);
+    F.phi[i4(d, i, j, k)] = ph;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const int dst = ODD ? c + (EX[q] + d.X1 * (EY[q] + d.Y1 * EZ[q])) : c;
+        const int slot = ODD ? OPP[q] : q;
+        F.pdf[dst + S1 * slot] = o1[q];
+        F.pdf[dst + S1 * (slot + 19)] = o2[q];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Periodic ghost copies of populations and phi, all enabled axes in one launch.  The
+// reference runs Z, Y and ZY kernels in sequence; their reads and writes are disjoint, so
+// one launch gives the same result.  Thread ranges: [0, nx*ny) z faces, [.., +nx*nz) y
+// faces, [.., +nx) y-z edges.
+// ---------------------------------------------------------------------------------------
+template <typename T, bool ODD>
+__global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jper) {
+    const Dims& d = F.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    int t = blockIdx.x * 256 + threadIdx.x;
+    const int64_t S1 = d.S1;
+    auto P = [&](int a, int b, int c) -> T& { return F.phi[i4(d, a, b, c)]; };
+    auto cp = [&](int dst, int src, int slot) {
+        F.pdf[dst + S1 * slot] = F.pdf[src + S1 * slot];
+        F.pdf[dst + S1 * (slot + 19)] = F.pdf[src + S1 * (slot + 19)];
+    };
+    const int nzf = kper ? nx * ny : 0, nyf = jper ? nx * nz : 0, ne = (kper && jper) ? nx : 0;
+    if (t < nzf) {
+        const int i = t % nx + 1, j = t / nx + 1;
+        const int up[5] = {6, 14, 13, 18, 17}, dn[5] = {5, 11, 12, 15, 16};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if (!ODD) {
+                cp(i1(d, i, j, nz + 1), i1(d, i, j, 1), up[m]);
+                cp(i1(d, i, j, 0), i1(d, i, j, nz), dn[m]);
+            } else {
+                cp(i1(d, i, j, 1), i1(d, i, j, nz + 1), up[m]);
+                cp(i1(d, i, j, nz), i1(d, i, j, 0), dn[m]);
+            }
+        }
+#pragma unroll
+        for (int kk = 1; kk <= 4; ++kk) {
+            P(i, j, kk + nz) = P(i, j, kk);
+            P(i, j, kk - 4) = P(i, j, nz + kk - 4);
+        }
+        return;
+    }
+    t -= nzf;
+    if (t < nyf) {
+        const int i = t % nx + 1, k = t / nx + 1;
+        const int up[5] = {4, 10, 9, 16, 18}, dn[5] = {3, 7, 8, 15, 17};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            if (!ODD) {
+                cp(i1(d, i, ny + 1, k), i1(d, i, 1, k), up[m]);
+                cp(i1(d, i, 0, k), i1(d, i, ny, k), dn[m]);
+            } else {
+                cp(i1(d, i, 1, k), i1(d, i, ny + 1, k), up[m]);
+                cp(i1(d, i, ny, k), i1(d, i, 0, k), dn[m]);
+            }
+        }
+#pragma unroll
+        for (int jj = 1; jj <= 4; ++jj) {
+            P(i, jj + ny, k) = P(i, jj, k);
+            P(i, jj - 4, k) = P(i, ny + jj - 4, k);
+        }
+        return;
+    }
+    t -= nyf;
+    if (t < ne) {
+        const int i = t + 1;
+        if (!ODD) {
+            cp(i1(d, i, ny + 1, nz + 1), i1(d, i, 1, 1), 18);
+            cp(i1(d, i, ny + 1, 0), i1(d, i, 1, nz), 16);
+            cp(i1(d, i, 0, nz + 1), i1(d, i, ny, 1), 17);
+            cp(i1(d, i, 0, 0), i1(d, i, ny, nz), 15);
+        } else {
+            cp(i1(d, i, 1, 1), i1(d, i, ny + 1, nz + 1), 18);
+            cp(i1(d, i, 1, nz), i1(d, i, ny + 1, 0), 16);
+            cp(i1(d, i, ny, 1), i1(d, i, 0, nz + 1), 17);
+            cp(i1(d, i, ny, nz), i1(d, i, 0, 0), 15);
+        }
+        for (int kk = 1; kk <= 4; ++kk)
+            for (int jj = 1; jj <= 4; ++jj) {
+                P(i, jj - 4, kk - 4) = P(i, ny + jj - 4, nz + kk - 4);
+                P(i, jj + ny, kk - 4) = P(i, jj, nz + kk - 4);
+                P(i, jj + ny, kk + nz) = P(i, jj, kk);
+                P(i, jj - 4, kk + nz) = P(i, ny + jj - 4, kk);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Zou-He pressure inlet (k = 1) and outlet (k = nz), one thread per (i, j).  The inlet
+// writes plane k <= 1 and the outlet plane k >= nz, so one launch serves both (nz >= 3).
+// Sums follow the slot order of each reference variant (the orders differ between the
+// before-odd and after-odd kernels and so does the rounding).
+// ---------------------------------------------------------------------------------------
+template <typename T, bool AFTER_ODD>
+__global__ __launch_bounds__(256) void k_zou_he(Fields<T> F, StepConst<T> P, int do_in, int do_out) {
+    const Dims& d = F.d;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= d.nx * d.ny) return;
+    const int i = t % d.nx + 1, j = t / d.nx + 1, nz = d.nz;
+    const int64_t S1 = d.S1;
+    const T c3 = T(0.333333333333333333), c6 = T(0.166666666666666667);
+    auto f = [&](int a, int b, int c, int e, int g) -> T& { return F.pdf[i1(d, a, b, c) + S1 * (e + 19 * g)]; };
+    if (do_in) {
+        const int w = F.walls[i2(d, i, j, 1)];
+        T& p0 = F.phi[i4(d, i, j, 0)];
+        p0 = P.phi_inlet * (1 - w) + p0 * w;
+        const T pv = p0;
+        F.phi[i4(d, i, j, -1)] = pv;
+        F.phi[i4(d, i, j, -2)] = pv;
+        F.phi[i4(d, i, j, -3)] = pv;
+        T rT[2];
+        rT[1] = P.rho_in;
+        rT[0] = P.rho_in * P.sa_inject;
+        rT[1] = rT[1] - rT[0];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            if (!AFTER_ODD) {
+                T tt = (rT[g] - (f(i, j, 1, 0, g) + f(i - 1, j, 1, 1, g) + f(i + 1, j, 1, 2, g) + f(i, j - 1, 1, 3, g) +
+                                 f(i, j + 1, 1, 4, g) + f(i - 1, j - 1, 1, 7, g) + f(i + 1, j - 1, 1, 8, g) +
+                                 f(i - 1, j + 1, 1, 9, g) + f(i + 1, j + 1, 1, 10, g) +
+                                 T(2.0) * (f(i, j, 2, 6, g) + f(i + 1, j, 2, 14, g) + f(i - 1, j, 2, 13, g) +
+                                           f(i, j + 1, 2, 18, g) + f(i, j - 1, 2, 17, g)))) * P.relaxation;
+                T tnx = T(0.5) * (f(i - 1, j, 1, 1, g) + f(i - 1, j - 1, 1, 7, g) + f(i - 1, j + 1, 1, 9, g) -
+                                  (f(i + 1, j, 1, 2, g) + f(i + 1, j - 1, 1, 8, g) + f(i + 1, j + 1, 1, 10, g)));
+                T tny = T(0.5) * (f(i, j - 1, 1, 3, g) + f(i - 1, j - 1, 1, 7, g) + f(i + 1, j - 1, 1, 8, g) -
+                                  (f(i, j + 1, 1, 4, g) + f(i + 1, j + 1, 1, 10, g) + f(i - 1, j + 1, 1, 9, g)));
+                f(i, j, 0, 5, g) = (f(i, j, 2, 6, g) + c3 * tt) * (1 - w) + f(i, j, 0, 5, g) * w;
+                f(i - 1, j, 0, 11, g) = (f(i + 1, j, 2, 14, g) + c6 * tt - tnx) * (1 - w) + f(i - 1, j, 0, 11, g) * w;
+                f(i + 1, j, 0, 12, g) = (f(i - 1, j, 2, 13, g) + c6 * tt + tnx) * (1 - w) + f(i + 1, j, 0, 12, g) * w;
+                f(i, j - 1, 0, 15, g) = (f(i, j + 1, 2, 18, g) + c6 * tt - tny) * (1 - w) + f(i, j - 1, 0, 15, g) * w;
+                f(i, j + 1, 0, 16, g) = (f(i, j - 1, 2, 17, g) + c6 * tt + tny) * (1 - w) + f(i, j + 1, 0, 16, g) * w;
+            } else {
+                T tt = (rT[g] - (f(i, j, 1, 0, g) + f(i, j, 1, 2, g) + f(i, j, 1, 1, g) + f(i, j, 1, 4, g) +
+                                 f(i, j, 1, 3, g) + f(i, j, 1, 8, g) + f(i, j, 1, 7, g) + f(i, j, 1, 10, g) +
+                                 f(i, j, 1, 9, g) +
+                                 T(2.0) * (f(i, j, 1, 5, g) + f(i, j, 1, 11, g) + f(i, j, 1, 12, g) +
+                                           f(i, j, 1, 15, g) + f(i, j, 1, 16, g)))) * P.relaxation;
+                T tnx = T(0.5) * (f(i, j, 1, 2, g) + f(i, j, 1, 8, g) + f(i, j, 1, 10, g) -
+                                  (f(i, j, 1, 1, g) + f(i, j, 1, 7, g) + f(i, j, 1, 9, g)));
+                T tny = T(0.5) * (f(i, j, 1, 4, g) + f(i, j, 1, 9, g) + f(i, j, 1, 10, g) -
+                                  (f(i, j, 1, 3, g) + f(i, j, 1, 8, g) + f(i, j, 1, 7, g)));
+                f(i, j, 1, 6, g) = (f(i, j, 1, 5, g) + c3 * tt) * (1 - w) + f(i, j, 1, 6, g) * w;
+                f(i, j, 1, 13, g) = (f(i, j, 1, 12, g) + c6 * tt + tnx) * (1 - w) + f(i, j, 1, 13, g) * w;
+                f(i, j, 1, 14, g) = (f(i, j, 1, 11, g) + c6 * tt - tnx) * (1 - w) + f(i, j, 1, 14, g) * w;
+                f(i, j, 1, 17, g) = (f(i, j, 1, 16, g) + c6 * tt + tny) * (1 - w) + f(i, j, 1, 17, g) * w;
+                f(i, j, 1, 18, g) = (f(i, j, 1, 15, g) + c6 * tt - tny) * (1 - w) + f(i, j, 1, 18, g) * w;
+            }
+        }
+    }
+    if (do_out) {
+        const int w = F.walls[i2(d, i, j, nz)];
+        const T ph = F.phi[i4(d, i, j, nz)];
+#pragma unroll
+        for (int m = 1; m <= 4; ++m) F.phi[i4(d, i, j, nz + m)] = ph;
+        T t1;
+        if (!AFTER_ODD) {
+            t1 = f(i, j, nz, 0, 0) + f(i - 1, j, nz, 1, 0) + f(i + 1, j, nz, 2, 0) + f(i, j - 1, nz, 3, 0) +
+                 f(i, j + 1, nz, 4, 0) + f(i - 1, j - 1, nz, 7, 0) + f(i + 1, j - 1, nz, 8, 0) +
+                 f(i - 1, j + 1, nz, 9, 0) + f(i + 1, j + 1, nz, 10, 0) +
+                 T(2.0) * (f(i, j, nz - 1, 5, 0) + f(i - 1, j, nz - 1, 11, 0) + f(i + 1, j, nz - 1, 12, 0) +
+                           f(i, j - 1, nz - 1, 15, 0) + f(i, j + 1, nz - 1, 16, 0)) +
+                 f(i, j, nz, 0, 1) + f(i - 1, j, nz, 1, 1) + f(i + 1, j, nz, 2, 1) + f(i, j - 1, nz, 3, 1) +
+                 f(i, j + 1, nz, 4, 1) + f(i - 1, j - 1, nz, 7, 1) + f(i + 1, j - 1, nz, 8, 1) +
+                 f(i - 1, j + 1, nz, 9, 1) + f(i + 1, j + 1, nz, 10, 1) +
+                 T(2.0) * (f(i, j, nz - 1, 5, 1) + f(i - 1, j, nz - 1, 11, 1) + f(i + 1, j, nz - 1, 12, 1) +
+                           f(i, j - 1, nz - 1, 15, 1) + f(i, j + 1, nz - 1, 16, 1)) -
+                 P.rho_out;
+        } else {
+            t1 = f(i, j, nz, 0, 0) + f(i, j, nz, 2, 0) + f(i, j, nz, 1, 0) + f(i, j, nz, 4, 0) + f(i, j, nz, 3, 0) +
+                 f(i, j, nz, 8, 0) + f(i, j, nz, 7, 0) + f(i, j, nz, 10, 0) + f(i, j, nz, 9, 0) +
+                 T(2.0) * (f(i, j, nz, 6, 0) + f(i, j, nz, 14, 0) + f(i, j, nz, 13, 0) + f(i, j, nz, 18, 0) +
+                           f(i, j, nz, 17, 0)) +
+                 f(i, j, nz, 0, 1) + f(i, j, nz, 2, 1) + f(i, j, nz, 1, 1) + f(i, j, nz, 4, 1) + f(i, j, nz, 3, 1) +
+                 f(i, j, nz, 8, 1) + f(i, j, nz, 7, 1) + f(i, j, nz, 10, 1) + f(i, j, nz, 9, 1) +
+                 T(2.0) * (f(i, j, nz, 6, 1) + f(i, j, nz, 14, 1) + f(i, j, nz, 13, 1) + f(i, j, nz, 18, 1) +
+                           f(i, j, nz, 17, 1)) -
+                 P.rho_out;
+        }
+        T t2 = t1 * T(0.5) * (T(1.0) - ph);
+        t1 = t1 - t2;
+        const T tt2[2] = {t1, t2};
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const T tt = tt2[g];
+            if (!AFTER_ODD) {
+                T tnx = T(0.5) * (f(i - 1, j, nz, 1, g) + f(i - 1, j - 1, nz, 7, g) + f(i - 1, j + 1, nz, 9, g) -
+                                  (f(i + 1, j, nz, 2, g) + f(i + 1, j - 1, nz, 8, g) + f(i + 1, j + 1, nz, 10, g)));
+                T tny = T(0.5) * (f(i, j - 1, nz, 3, g) + f(i - 1, j - 1, nz, 7, g) + f(i + 1, j - 1, nz, 8, g) -
+                                  (f(i, j + 1, nz, 4, g) + f(i + 1, j + 1, nz, 10, g) + f(i - 1, j + 1, nz, 9, g)));
+                f(i, j, nz + 1, 6, g) = (f(i, j, nz - 1, 5, g) - c3 * tt) * (1 - w) + f(i, j, nz + 1, 6, g) * w;
+                f(i - 1, j, nz + 1, 13, g) = (f(i + 1, j, nz - 1, 12, g) - c6 * tt - tnx) * (1 - w) + f(i - 1, j, nz + 1, 13, g) * w;
+                f(i + 1, j, nz + 1, 14, g) = (f(i - 1, j, nz - 1, 11, g) - c6 * tt + tnx) * (1 - w) + f(i + 1, j, nz + 1, 14, g) * w;
+                f(i, j - 1, nz + 1, 17, g) = (f(i, j + 1, nz - 1, 16, g) - c6 * tt - tny) * (1 - w) + f(i, j - 1, nz + 1, 17, g) * w;
+                f(i, j + 1, nz + 1, 18, g) = (f(i, j - 1, nz - 1, 15, g) - c6 * tt + tny) * (1 - w) + f(i, j + 1, nz + 1, 18, g) * w;
+            } else {
+                T tnx = T(0.5) * (f(i, j, nz, 2, g) + f(i, j, nz, 8, g) + f(i, j, nz, 10, g) -
+                                  (f(i, j, nz, 1, g) + f(i, j, nz, 7, g) + f(i, j, nz, 9, g)));
+                T tny = T(0.5) * (f(i, j, nz, 4, g) + f(i, j, nz, 10, g) + f(i, j, nz, 9, g) -
+                                  (f(i, j, nz, 3, g) + f(i, j, nz, 7, g) + f(i, j, nz, 8, g)));
+                f(i, j, nz, 5, g) = (f(i, j, nz, 6, g) - c3 * tt) * (1 - w) + f(i, j, nz, 5, g) * w;
+                f(i, j, nz, 11, g) = (f(i, j, nz, 14, g) - c6 * tt + tnx) * (1 - w) + f(i, j, nz, 11, g) * w;
+                f(i, j, nz, 12, g) = (f(i, j, nz, 13, g) - c6 * tt - tnx) * (1 - w) + f(i, j, nz, 12, g) * w;
+                f(i, j, nz, 15, g) = (f(i, j, nz, 18, g) - c6 * tt + tny) * (1 - w) + f(i, j, nz, 15, g) * w;
+                f(i, j, nz, 16, g) = (f(i, j, nz, 17, g) - c6 * tt - tny) * (1 - w) + f(i, j, nz, 16, g) * w;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Colour-gradient chain
+// ---------------------------------------------------------------------------------------
+// ISO4 central-difference stencil along one axis, summed in the reference order
+// (src/main_iteration_GPU.cu:765-791, 914-996).
+template <typename T, int AX>
+__device__ __forceinline__ T iso4(const T* __restrict__ a, int c, int sx, int sy, int sz) {
+    // offsets of the 4 "plus" terms of the second group, per axis
+    constexpr int G2[3][4][3] = {{{1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {1, 0, -1}},
+                                 {{1, 1, 0}, {-1, 1, 0}, {0, 1, 1}, {0, 1, -1}},
+                                 {{1, 0, 1}, {-1, 0, 1}, {0, 1, 1}, {0, -1, 1}}};
+    const int o1 = AX == 0 ? sx : (AX == 1 ? sy : sz);
+    const T d1 = a[c + o1] - a[c - o1];
+    T acc = T(0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int o = G2[AX][m][0] * sx + G2[AX][m][1] * sy + G2[AX][m][2] * sz;
+        acc = (m == 0) ? a[c + o] : acc + a[c + o];
+        acc = acc - a[c - o];
+    }
+    return (T(1.0) / T(6.0)) * d1 + (T(1.0) / T(12.0)) * acc;
+}
+
+// K3: phi at solid-boundary nodes = D3Q19-weighted mean over fluid neighbours, [-2, n+3].
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_extrap(Fields<T> F, const uint32_t* __restrict__ list, int n) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c = i4(d, i, j, k);
+    T ps = T(0), ws = T(0);
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const int nb = c + EX[q] + d.X4 * (EY[q] + d.Y4 * EZ[q]);
+        if (F.wtype[nb] <= 0) {
+            ps += F.phi[nb] * wq<T>(q);
+            ws += wq<T>(q);
+        }
+    }
+    F.phi[c] = ps / ws;
+}
+
+// K4 + K5: interface normals from phi (ISO4), then the geometrical wetting correction at
+// fluid-boundary nodes, over fluid nodes in [-1, n+2].  Solid nodes keep cn = c_norm = 0
+// from initialisation (the reference rewrites those zeros every step; K6 then overwrites
+// the solid-boundary normals in [0, n+1] from fluid neighbours only).
+template <typename T>
+__global__ __launch_bounds__(256) void k_normals(Fields<T> F, StepConst<T> P, const uint32_t* __restrict__ list, int n) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c4 = i4(d, i, j, k), c2 = i2(d, i, j, k);
+    const int sx = 1, sy = d.X4, sz = d.X4 * d.Y4;
+    const T cx = iso4<T, 0>(F.phi, c4, sx, sy, sz);
+    const T cy = iso4<T, 1>(F.phi, c4, sx, sy, sz);
+    const T cz = iso4<T, 2>(F.phi, c4, sx, sy, sz);
+    const T cn = sqrt(cx * cx + cy * cy + cz * cz);
+    if (cn < T(1e-6)) {
+        F.cnx[c2] = T(0); F.cny[c2] = T(0); F.cnz[c2] = T(0); F.cnorm[c2] = T(0);
+        return;
+    }
+    T vx0 = cx / cn, vy0 = cy / cn, vz0 = cz / cn;
+    F.cnorm[c2] = cn;
+    if (F.wtype[c4] == -1 && cn > T(1e-6)) {
+        // src/main_iteration_GPU.cu:809-878
+        const T lambda = T(0.5), leps = T(1e-6), ct = P.cos_theta;
+        const T nwx = F.snx[c4], nwy = F.sny[c4], nwz = F.snz[c4];
+        T vx1 = vx0 - lambda * (vx0 + nwx);
+        T vy1 = vy0 - lambda * (vy0 + nwy);
+        T vz1 = vz0 - lambda * (vz0 + nwz);
+        T e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - ct;
+        if ((fabs(vx0 + nwx) + fabs(vy0 + nwy) + fabs(vz0 + nwz) > leps ||
+             fabs(vx0 - nwx) + fabs(vy0 - nwy) + fabs(vz0 - nwz) > leps) && e0 > leps) {
+            T e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
+            T tt = T(1.0) / (e1 - e0);
+            T vx2 = tt * (vx0 * e1 - vx1 * e0);
+            T vy2 = tt * (vy0 * e1 - vy1 * e0);
+            T vz2 = tt * (vz0 * e1 - vz1 * e0);
+            T e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
+            if (e2 > leps) {
+                for (int it = 2; it <= 4; ++it) {
+                    vx0 = vx1; vy0 = vy1; vz0 = vz1;
+                    vx1 = vx2; vy1 = vy2; vz1 = vz2;
+                    e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - sqrt(vx0 * vx0 + vy0 * vy0 + vz0 * vz0) * ct;
+                    e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
+                    tt = T(1.0) / (e1 - e0);
+                    if (isinf(tt)) break;
+                    vx2 = tt * (vx0 * e1 - vx1 * e0);
+                    vy2 = tt * (vy0 * e1 - vy1 * e0);
+                    vz2 = tt * (vz0 * e1 - vz1 * e0);
+                    e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
+                }
+            }
+            tt = T(1.0) / (T(1e-30) + sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2));
+            vx0 = vx2 * tt; vy0 = vy2 * tt; vz0 = vz2 * tt;
+        }
+    }
+    F.cnx[c2] = vx0; F.cny[c2] = vy0; F.cnz[c2] = vz0;
+}
+
+// K6: normals at solid-boundary nodes in [0, n+1] from fluid neighbours.
+template <typename T>
+__global__ __launch_bounds__(256) void k_normal_extrap(Fields<T> F, const uint32_t* __restrict__ list, int n) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    if (i < 0 || j < 0 || k < 0 || i > d.nx + 1 || j > d.ny + 1 || k > d.nz + 1) return;
+    const int c4 = i4(d, i, j, k), c2 = i2(d, i, j, k);
+    T sx = T(0), sy = T(0), sz = T(0), ws = T(0);
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        if (F.wtype[c4 + EX[q] + d.X4 * (EY[q] + d.Y4 * EZ[q])] <= 0) {
+            const int nb = c2 + EX[q] + d.X2 * (EY[q] + d.Y2 * EZ[q]);
+            sx += F.cnx[nb] * wq<T>(q);
+            sy += F.cny[nb] * wq<T>(q);
+            sz += F.cnz[nb] * wq<T>(q);
+            ws += wq<T>(q);
+        }
+    }
+    F.cnx[c2] = sx / ws; F.cny[c2] = sy / ws; F.cnz[c2] = sz / ws;
+}
+
+// K7: curvature from the ISO4 derivatives of the normals, at interior fluid nodes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_curvature(Fields<T> F, const uint32_t* __restrict__ list, int n) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c2 = i2(d, i, j, k);
+    const int sx = 1, sy = d.X2, sz = d.X2 * d.Y2;
+    const T kxx = iso4<T, 0>(F.cnx, c2, sx, sy, sz);
+    const T kyy = iso4<T, 1>(F.cny, c2, sx, sy, sz);
+    const T kzz = iso4<T, 2>(F.cnz, c2, sx, sy, sz);
+    const T kxy = iso4<T, 1>(F.cnx, c2, sx, sy, sz);
+    const T kxz = iso4<T, 2>(F.cnx, c2, sx, sy, sz);
+    const T kyx = iso4<T, 0>(F.cny, c2, sx, sy, sz);
+    const T kyz = iso4<T, 2>(F.cny, c2, sx, sy, sz);
+    const T kzx = iso4<T, 0>(F.cnz, c2, sx, sy, sz);
+    const T kzy = iso4<T, 1>(F.cnz, c2, sx, sy, sz);
+    const T cx = F.cnx[c2], cy = F.cny[c2], cz = F.cnz[c2];
+    F.curv[i1(d, i, j, k)] = (cx * cx - T(1.0)) * kxx + (cy * cy - T(1.0)) * kyy + (cz * cz - T(1.0)) * kzz +
+                             cx * cy * (kxy + kyx) + cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+}
+
+// ---------------------------------------------------------------------------------------
+// Initial state (src/Init_multiphase.cpp:299-443)
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_init_phi(Fields<T> F, int option, double z0, double phi_inlet, int open_z) {
+    const Dims& d = F.d;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= d.S4) return;
+    const int i = (int)(t % d.X4) - 3, j = (int)((t / d.X4) % d.Y4) - 3, k = (int)(t / ((int64_t)d.X4 * d.Y4)) - 3;
+    double v = 0.0;
+    if (i >= 0 && j >= 0 && k >= 0 && i <= d.nx + 1 && j <= d.ny + 1 && k <= d.nz + 1) {
+        const double x = i, y = j, z = k;
+        if (option == 1) v = (z <= z0) ? 1.0 : -1.0;
+        else if (option == 2) v = (z <= z0) ? -1.0 : 1.0;
+        else {
+            const double cx = (option == 5) ? (double)(d.nx + 1) * 0.5 : (double)(d.nx + 1) * 0.0;
+            const double dx = x - cx, dz = z - (double)(d.nz + 1) * 0.5, dy = y - (double)(d.ny + 1) * 0.5;
+            const bool inside = dx * dx + dz * dz + dy * dy <= z0 * z0;
+            v = (option == 4) ? (inside ? -1.0 : 1.0) : (inside ? 1.0 : -1.0);
+        }
+    }
+    if (open_z && (double)k <= 0) v = phi_inlet;
+    F.phi[t] = (T)v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_init_eq(Fields<T> F, const double* rho, const double* u, const double* v,
+                                                 const double* w) {
+    const Dims& d = F.d;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= d.S1) return;
+    const int i = t % d.X1, j = (t / d.X1) % d.Y1, k = t / (d.X1 * d.Y1);
+    const T uu = u ? (T)u[t] : T(0), vv = v ? (T)v[t] : T(0), ww = w ? (T)w[t] : T(0), rr = rho ? (T)rho[t] : T(1);
+    const T usq = uu * uu + vv * vv + ww * ww;
+    const T ph = F.phi[i4(d, i, j, k)];
+    const T rc[2] = {rr * (T(1.0) + ph) * T(0.5), rr * (T(1.0) - ph) * T(0.5)};
+    const T cu[19] = {T(0), uu, -uu, vv, -vv, ww, -ww, uu + vv, -uu + vv, uu - vv, -uu - vv,
+                      uu + ww, -uu + ww, uu - ww, -uu - ww, vv + ww, -vv + ww, vv - ww, -vv - ww};
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const T r = rc[g];
+        F.pdf[t + d.S1 * (19 * g)] = r * W<T>::w0 + r * W<T>::w0 * (T(-1.5) * usq);
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            const T wqq = q <= 6 ? W<T>::w1 : W<T>::w2;
+            F.pdf[t + d.S1 * (q + 19 * g)] = r * wqq + r * wqq * (T(3.0) * cu[q] + T(4.5) * cu[q] * cu[q] - T(1.5) * usq);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// compute_macro_vars (src/Misc.cpp:222-274) and the monitor sums (src/Monitor.cpp:17-80)
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void macro_node(const Fields<T>& F, const StepConst<T>& P, int i, int j, int k, double& rho,
+                                           double& u, double& v, double& w) {
+    const Dims& d = F.d;
+    const int c = i1(d, i, j, k), c2 = i2(d, i, j, k);
+    T ft[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) ft[q] = F.pdf[c + d.S1 * q] + F.pdf[c + d.S1 * (q + 19)];
+    T r = ft[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) r = r + ft[q];
+    const T tmp = T(0.5) * P.gamma * F.curv[c] * F.cnorm[c2];
+    const T fx = tmp * F.cnx[c2], fy = tmp * F.cny[c2], fz = tmp * F.cnz[c2] + P.force_z;
+    rho = (double)r;
+    u = (double)(ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] - T(0.5) * fx);
+    v = (double)(ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] - T(0.5) * fy);
+    w = (double)(ft[5] - ft[6] + ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18] - T(0.5) * fz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_macro(Fields<T> F, StepConst<T> P, double* rho, double* u, double* v, double* w) {
+    const Dims& d = F.d;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= d.nx * d.ny * d.nz) return;
+    const int i = t % d.nx + 1, j = (t / d.nx) % d.ny + 1, k = t / (d.nx * d.ny) + 1;
+    const int c = i1(d, i, j, k);
+    double r = 0, uu = 0, vv = 0, ww = 0;
+    if (F.walls[i2(d, i, j, k)] == 0) macro_node(F, P, i, j, k, r, uu, vv, ww);
+    if (rho) rho[c] = r;
+    if (u) u[c] = uu;
+    if (v) v[c] = vv;
+    if (w) w[c] = ww;
+}
+
+// Partial monitor sums: grid (nsplit, nz); block (b, k) reduces rows [b*rows, (b+1)*rows)
+// of slice k into 10 FP64 values: fl1 fl2 vol1 vol2 mass1 mass2 pre umax2 usq1 usq2.
+constexpr int MON_VALUES = 10;
+template <typename T>
+__global__ __launch_bounds__(256) void k_monitor_partial(Fields<T> F, StepConst<T> P, int rows, double* __restrict__ part) {
+    const Dims& d = F.d;
+    const int k = blockIdx.y + 1;
+    const int j0 = blockIdx.x * rows + 1, j1 = min(j0 + rows - 1, d.ny);
+    double acc[MON_VALUES];
+#pragma unroll
+    for (int m = 0; m < MON_VALUES; ++m) acc[m] = 0.0;
+    const int cnt = (j1 - j0 + 1) * d.nx;
+    for (int t = threadIdx.x; t < cnt; t += 256) {
+        const int i = t % d.nx + 1, j = j0 + t / d.nx;
+        if (F.walls[i2(d, i, j, k)] != 0) continue;
+        double r, u, v, w;
+        macro_node(F, P, i, j, k, r, u, v, w);
+        const double ph = (double)F.phi[i4(d, i, j, k)];
+        // per-node terms in the reference's operand order (src/Monitor.cpp:60-67)
+        acc[0] += w * 0.5 * (1.0 + ph);
+        acc[1] += w * 0.5 * (1.0 - ph);
+        acc[2] += 0.5 * (1.0 + ph);
+        acc[3] += 0.5 * (1.0 - ph);
+        acc[4] += r * 0.5 * (1.0 + ph);
+        acc[5] += r * 0.5 * (1.0 - ph);
+        acc[6] += r;
+        const double u2 = u * u + v * v + w * w;
+        acc[7] = fmax(acc[7], u2);
+        if (ph > 0.999) acc[8] += u2;
+        else if (ph < -0.999) acc[9] += u2;
+    }
+    __shared__ double red[MON_VALUES][256];
+#pragma unroll
+    for (int m = 0; m < MON_VALUES; ++m) red[m][threadIdx.x] = acc[m];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int m = 0; m < MON_VALUES; ++m)
+                red[m][threadIdx.x] = (m == 7) ? fmax(red[m][threadIdx.x], red[m][threadIdx.x + s])
+                                               : red[m][threadIdx.x] + red[m][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < MON_VALUES) part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MON_VALUES + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Combine the partials of each slice in block order: out[m * nz + k].
+template <typename T>
+__global__ void k_monitor_combine(const double* __restrict__ part, int nsplit, int nz, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nz) return;
+    for (int m = 0; m < MON_VALUES; ++m) {
+        double s = part[((int64_t)k * nsplit) * MON_VALUES + m];
+        for (int b = 1; b < nsplit; ++b) {
+            const double x = part[((int64_t)k * nsplit + b) * MON_VALUES + m];
+            s = (m == 7) ? fmax(s, x) : s + x;
+        }
+        out[m * nz + k] = s;
+    }
+}
+
+}  // namespace dev
+}  // namespace mflbm

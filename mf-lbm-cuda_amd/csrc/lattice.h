@@ -1,0 +1,92 @@
+// Lattice tables, index arithmetic and device-state descriptors shared by the HIP kernels
+// and the host side of libmflbm.  D3Q19 velocity set and weights as in
+// includes/Module.h:98-122 of the reference; ghost-aware 1-based indexing as in
+// includes/Idx_gpu.cuh:49-70.
+#pragma once
+#include <cstdint>
+
+#define MFLBM_HD __host__ __device__ __forceinline__
+
+namespace mflbm {
+
+constexpr int EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int EY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int EZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+// Grid extents.  All node indices fit in 32 bits for single domains up to ~1000^3;
+// slot offsets (q * S1) are formed in 64 bits.
+struct Dims {
+    int nx, ny, nz;
+    int X1, Y1, Z1;  // nx+2 ...
+    int X2, Y2, Z2;  // nx+4 ...
+    int X4, Y4, Z4;  // nx+8 ...
+    int64_t S1, S2, S4;
+};
+
+inline Dims make_dims(int nx, int ny, int nz) {
+    Dims d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.X1 = nx + 2; d.Y1 = ny + 2; d.Z1 = nz + 2;
+    d.X2 = nx + 4; d.Y2 = ny + 4; d.Z2 = nz + 4;
+    d.X4 = nx + 8; d.Y4 = ny + 8; d.Z4 = nz + 8;
+    d.S1 = (int64_t)d.X1 * d.Y1 * d.Z1;
+    d.S2 = (int64_t)d.X2 * d.Y2 * d.Z2;
+    d.S4 = (int64_t)d.X4 * d.Y4 * d.Z4;
+    return d;
+}
+
+// i, j, k are 1-based interior coordinates (0 and n+1 are the first ghost layer).
+MFLBM_HD int i1(const Dims& d, int i, int j, int k) { return i + d.X1 * (j + d.Y1 * k); }
+MFLBM_HD int i2(const Dims& d, int i, int j, int k) { return (i + 1) + d.X2 * ((j + 1) + d.Y2 * (k + 1)); }
+MFLBM_HD int i4(const Dims& d, int i, int j, int k) { return (i + 3) + d.X4 * ((j + 3) + d.Y4 * (k + 3)); }
+
+// Node lists store packed coordinates: (i+4) | (j+4) << 10 | (k+4) << 21.
+constexpr int PACK_BIAS = 4;
+MFLBM_HD uint32_t pack_ijk(int i, int j, int k) {
+    return (uint32_t)(i + PACK_BIAS) | ((uint32_t)(j + PACK_BIAS) << 10) | ((uint32_t)(k + PACK_BIAS) << 21);
+}
+MFLBM_HD void unpack_ijk(uint32_t p, int& i, int& j, int& k) {
+    i = (int)(p & 1023u) - PACK_BIAS;
+    j = (int)((p >> 10) & 2047u) - PACK_BIAS;
+    k = (int)(p >> 21) - PACK_BIAS;
+}
+
+template <typename T>
+struct Fields {
+    Dims d;
+    T* pdf;      // 38 * S1
+    T* phi;      // S4
+    T* cnx;      // S2
+    T* cny;
+    T* cnz;
+    T* cnorm;    // S2
+    T* curv;     // S1
+    const T* snx;  // S4
+    const T* sny;
+    const T* snz;
+    const uint8_t* walls;  // S2, 0/1
+    const int8_t* wtype;   // S4, -1/0/1/2
+};
+
+template <typename T>
+struct StepConst {
+    T gamma, force_z, nui1, nui2, beta, phi_inlet, sa_inject, cos_theta, rho_in, rho_out, relaxation;
+};
+
+struct NodeLists {
+    const uint32_t* fluid;      // walls == 0 in [1,n]^3        : collide, curvature
+    int n_fluid;
+    const uint32_t* fluid_ext;  // walls == 0 in [-1,n+2]^3     : normals + wetting
+    int n_fluid_ext;
+    const uint32_t* solid_bnd;  // walls_type == 2 in [-2,n+3]^3: phi / normal extrapolation
+    int n_solid_bnd;
+};
+
+struct BoundaryFlags {
+    int kper, jper;
+    int open_z;      // kper == 0 and no z walls
+    int inlet_bc, outlet_bc;
+};
+
+}  // namespace mflbm

@@ -1,0 +1,50 @@
+"""Shared helpers of the test suite: workload construction and array comparison."""
+import numpy as np
+
+from mflbm import cases, config
+
+
+def small_drainage(n=(24, 24, 32), seed=7, radius=4.0, porosity=0.45, jper=1, theta=45.0, nu_ratio=1.0,
+                   body_force=5e-4, buffer=3):
+    """Sphere-pack drainage case with empty inlet/outlet buffers (reference axis layout)."""
+    nx, ny, nz = n
+    nu2 = cases.nu_from_tau(0.7)
+    ctrl = cases.drainage_control(n=n, theta=theta, nu1=nu_ratio * nu2, nu2=nu2, gamma=0.01,
+                                  body_force=body_force, interface_z0=buffer + 1.0, jper=jper)
+    geo = cases.sphere_geometry(n, radius, porosity, seed, jper=jper)
+    geo[:buffer] = 0
+    geo[nz - buffer:] = 0
+    return geo, config.derive(ctrl)
+
+
+def periodic_drop(n=(24, 24, 24), radius=6.0, theta=45.0, geo_seed=None, body_force=0.0):
+    """Droplet (option 5) in a y/z periodic box with x walls; optional sphere obstacles."""
+    ctrl = cases.periodic_control(n=n, body_force=body_force, theta=theta, option=5, interface_z0=radius)
+    nx, ny, nz = n
+    if geo_seed is None:
+        geo = np.zeros((nz, ny, nx), np.int8)
+    else:
+        geo = cases.geometry.sphere_pack((nz, ny, nx), 3.0, 0.85, geo_seed, periodic=(True, True, False))
+    return geo, config.derive(ctrl)
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def assert_same(a, b, tol=0.0, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} != {b.shape}"
+    if tol == 0.0:
+        bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+        if bad.any():
+            idx = np.argwhere(bad)[:5]
+            raise AssertionError(f"{what}: {bad.sum()} of {a.size} entries differ; first {idx.tolist()}; "
+                                 f"rel Linf {rel_linf(a, b):.3e}")
+    else:
+        r = rel_linf(a, b)
+        assert r <= tol, f"{what}: relative Linf {r:.3e} > {tol:.1e}"

@@ -1,0 +1,28 @@
+"""The C-ABI library loads and exports every symbol include/mflbm.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from mflbm import gpu
+
+
+def declared_symbols():
+    with open(gpu.HEADER) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(mflbm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    assert declared_symbols() == sorted(gpu.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(gpu.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header_layout():
+    # mflbm_params: 3 int64 + 11 int32 (+4 pad) + 11 double + 2 int32
+    assert ctypes.sizeof(gpu.Params) == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 2 * 4
+    assert ctypes.sizeof(gpu.MonitorOut) == 7 * 8 + 3 * 8 + 8

@@ -1,0 +1,143 @@
+"""GPU parity: the HIP step (through the C ABI) against the FP64 oracle on the same inputs.
+
+FP64 handles must agree bit for bit on every field the reference keeps (the 38
+population slots including ghosts, phi, normals, curvature at fluid nodes) — the
+north-star bound is a relative L-infinity of 1e-12, bit equality is stronger.  Monitor
+sums are reduced in a different order on the device; they must agree to 1e-12 of the
+summed magnitudes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_same, periodic_drop, rel_linf, small_drainage
+from mflbm import cases, config, gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def pair(geo, params, precision=gpu.FP64):
+    o = O.Oracle(params, geo)
+    g = gpu.Solver(params, geo, precision)
+    return o, g
+
+
+def fluid_interior_mask(o):
+    walls = o.int_array("walls")[1:-1, 1:-1, 1:-1]  # s2 -> s1 footprint
+    m = np.zeros(o.shape(1), bool)
+    m[1:-1, 1:-1, 1:-1] = walls[1:-1, 1:-1, 1:-1] == 0
+    return m
+
+
+def compare_geometry(o, g):
+    assert_same(g.download("walls").astype(np.int32), o.int_array("walls"), what="walls")
+    assert_same(g.download("walls_type").astype(np.int32), o.int_array("walls_type"), what="walls_type")
+    for n in ("s_nx", "s_ny", "s_nz"):
+        assert_same(g.download(n), o.array(n), what=n)
+
+
+def compare_state(o, g, tol=0.0, tag=""):
+    assert_same(g.download("pdf"), o.array("pdf"), tol, what=f"{tag} pdf")
+    assert_same(g.download("phi"), o.array("phi"), tol, what=f"{tag} phi")
+    for n in ("cn_x", "cn_y", "cn_z", "c_norm"):
+        assert_same(g.download(n), o.array(n), tol, what=f"{tag} {n}")
+    m = fluid_interior_mask(o)
+    assert_same(g.download("curv")[m], o.array("curv")[m], tol, what=f"{tag} curv(fluid)")
+
+
+CASES = {
+    "drainage_yper": lambda: small_drainage(n=(24, 24, 32), seed=7, jper=1),
+    "drainage_ywalls": lambda: small_drainage(n=(20, 24, 28), seed=11, jper=0, theta=30.0, nu_ratio=0.1),
+    "drop_periodic_yz": lambda: periodic_drop(n=(24, 20, 22), radius=6.0, geo_seed=3, body_force=1e-5),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_setup_and_steps_bitwise(name):
+    geo, p = CASES[name]()
+    say(f"case={name} shape={geo.shape} fp64 setup")
+    o, g = pair(geo, p)
+    compare_geometry(o, g)
+    assert o.counts()[0] >= 0
+    o.setup()
+    g.setup()
+    compare_state(o, g, tag=f"{name} init")
+    for t in range(1, 5):
+        say(f"case={name} step {t}")
+        o.step(t)
+        g.step(t)
+        compare_state(o, g, tag=f"{name} step{t}")
+    say(f"case={name} graph run 5..40")
+    o.run(5, 36)
+    g.run(5, 36)
+    compare_state(o, g, tag=f"{name} step40")
+
+
+def test_config1_1000_steps():
+    """64^3 drainage of the first workload, 1000 steps, relative L-inf <= 1e-12 (bitwise expected)."""
+    geo, ctrl = cases.config1()
+    p = config.derive(ctrl)
+    say("case=config1 64^3 fp64 1000 steps")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    o.run(1, 1000)
+    g.run(1, 1000)
+    for n in ("pdf", "phi"):
+        r = rel_linf(g.download(n), o.array(n))
+        assert r <= 1e-12, f"{n}: relative Linf {r:.3e}"
+    compare_state(o, g, tag="config1 step1000")
+    rho_o, u_o, v_o, w_o = o.macro(zero_solid_phi=False)
+    rho_g, u_g, v_g, w_g = g.macro()
+    for a, b, n in ((rho_g, rho_o, "rho"), (u_g, u_o, "u"), (v_g, v_o, "v"), (w_g, w_o, "w")):
+        assert rel_linf(a, b) <= 1e-12, n
+
+
+def test_monitor_matches_oracle():
+    geo, p = small_drainage(n=(24, 24, 32), seed=5, jper=1)
+    say("case=monitor fp64")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    o.run(1, 20)
+    g.run(1, 20)
+    rho, u, v, w = o.macro(zero_solid_phi=False)
+    ref = o.monitor_sums(rho, u, v, w)
+    got = g.monitor()
+    mag = {"fl1": np.abs(w).sum(), "fl2": np.abs(w).sum(), "vol1": rho.size, "vol2": rho.size,
+           "mass1": np.abs(rho).sum(), "mass2": np.abs(rho).sum(), "pre": np.abs(rho).sum()}
+    for k in ("fl1", "fl2", "vol1", "vol2", "mass1", "mass2", "pre"):
+        err = np.abs(got[k] - ref[k]).max()
+        assert err <= 1e-12 * mag[k], f"{k}: {err:.3e}"
+    assert got["umax2"] == ref["umax2"]
+    assert abs(got["usq1"] - ref["usq1"]) <= 1e-12 * max(1.0, abs(ref["usq1"]))
+    assert abs(got["usq2"] - ref["usq2"]) <= 1e-12 * max(1.0, abs(ref["usq2"]))
+
+
+def test_fp32_tracks_fp64():
+    """FP32 populations/collision against the FP64 oracle: saturation and mass stay close."""
+    geo, p = small_drainage(n=(24, 24, 32), seed=9, jper=1)
+    say("case=fp32 vs fp64 oracle")
+    o = O.Oracle(p, geo)
+    g = gpu.Solver(p, geo, gpu.FP32)
+    o.setup()
+    g.setup()
+    o.run(1, 200)
+    g.run(1, 200)
+    rho, u, v, w = o.macro(zero_solid_phi=False)
+    ref = o.monitor_sums(rho, u, v, w)
+    got = g.monitor()
+    s_ref = ref["vol1"].sum() / (ref["vol1"].sum() + ref["vol2"].sum())
+    s_got = got["vol1"].sum() / (got["vol1"].sum() + got["vol2"].sum())
+    assert abs(s_got - s_ref) < 1e-3, (s_got, s_ref)
+    m_ref = ref["mass1"].sum() + ref["mass2"].sum()
+    m_got = got["mass1"].sum() + got["mass2"].sum()
+    assert abs(m_got - m_ref) / m_ref < 1e-4
+    assert rel_linf(g.download("phi"), o.array("phi")) < 0.05
