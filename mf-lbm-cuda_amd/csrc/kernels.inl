@@ -220,9 +220,12 @@ __global__ __launch_bounds__(256) void k_collide(Fields<T> F, StepConst<T> P, co
 
 // ---------------------------------------------------------------------------------------
 // Periodic ghost copies of populations and phi, all enabled axes in one launch.  The
-// reference runs Z, Y and ZY kernels in sequence; their reads and writes are disjoint, so
-// one launch gives the same result.  Thread ranges: [0, nx*ny) z faces, [.., +nx*nz) y
-// faces, [.., +nx) y-z edges.
+// reference runs the Z, Y and ZY kernels in sequence.  After an odd step the Z-face, Y-face
+// and edge copies all write four edge slots ((j,k) = (1,1) slot 18, (1,nz) slot 16,
+// (ny,1) slot 17, (ny,nz) slot 15); the edge copy runs last and wins.  Here the face
+// threads leave those slots to the edge threads, which makes every remaining write
+// unique, and no thread reads a slot another thread writes, so one launch reproduces the
+// sequence.  Thread ranges: [0, nx*ny) z faces, [.., +nx*nz) y faces, [.., +nx) y-z edges.
 // ---------------------------------------------------------------------------------------
 template <typename T, bool ODD>
 __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jper) {
@@ -236,6 +239,7 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
         F.pdf[dst + S1 * (slot + 19)] = F.pdf[src + S1 * (slot + 19)];
     };
     const int nzf = kper ? nx * ny : 0, nyf = jper ? nx * nz : 0, ne = (kper && jper) ? nx : 0;
+    const bool edges = kper && jper;
     if (t < nzf) {
         const int i = t % nx + 1, j = t / nx + 1;
         const int up[5] = {6, 14, 13, 18, 17}, dn[5] = {5, 11, 12, 15, 16};
@@ -245,8 +249,10 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
                 cp(i1(d, i, j, nz + 1), i1(d, i, j, 1), up[m]);
                 cp(i1(d, i, j, 0), i1(d, i, j, nz), dn[m]);
             } else {
-                cp(i1(d, i, j, 1), i1(d, i, j, nz + 1), up[m]);
-                cp(i1(d, i, j, nz), i1(d, i, j, 0), dn[m]);
+                const bool edge_up = edges && ((j == 1 && up[m] == 18) || (j == ny && up[m] == 17));
+                const bool edge_dn = edges && ((j == 1 && dn[m] == 16) || (j == ny && dn[m] == 15));
+                if (!edge_up) cp(i1(d, i, j, 1), i1(d, i, j, nz + 1), up[m]);
+                if (!edge_dn) cp(i1(d, i, j, nz), i1(d, i, j, 0), dn[m]);
             }
         }
 #pragma unroll
@@ -266,8 +272,10 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
                 cp(i1(d, i, ny + 1, k), i1(d, i, 1, k), up[m]);
                 cp(i1(d, i, 0, k), i1(d, i, ny, k), dn[m]);
             } else {
-                cp(i1(d, i, 1, k), i1(d, i, ny + 1, k), up[m]);
-                cp(i1(d, i, ny, k), i1(d, i, 0, k), dn[m]);
+                const bool edge_up = edges && ((k == 1 && up[m] == 18) || (k == nz && up[m] == 16));
+                const bool edge_dn = edges && ((k == 1 && dn[m] == 17) || (k == nz && dn[m] == 15));
+                if (!edge_up) cp(i1(d, i, 1, k), i1(d, i, ny + 1, k), up[m]);
+                if (!edge_dn) cp(i1(d, i, ny, k), i1(d, i, 0, k), dn[m]);
             }
         }
 #pragma unroll
