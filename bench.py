@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cubic lattice edge")
     ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--probe-mib", type=int, default=2048, help="plain-copy calibration size (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU z-slab "
@@ -116,6 +117,13 @@ def main():
     step_bytes = n_fluid * 78 * s + n_all
     log(f"collide {coll_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), step (kernels) {step_ms:.3f} ms")
 
+    probe = None
+    if args.probe_mib > 0:
+        nbytes = args.probe_mib << 20
+        pms = sim.bandwidth_probe(nbytes, repeats=10)
+        probe = {"bytes": nbytes, "ms": round(pms, 4), "gbs": round(2 * nbytes / (pms * 1e-3) / 1e9, 1)}
+        log(f"plain copy of {args.probe_mib} MiB: {pms:.3f} ms = {probe['gbs']} GB/s (read+write)")
+
     cpu = None
     if args.cpu_steps > 0:
         log(f"CPU baseline: {args.cpu_steps} oracle steps")
@@ -136,6 +144,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_collide (odd+even average)"},
+        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

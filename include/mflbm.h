@@ -126,6 +126,11 @@ int64_t mflbm_count(const mflbm_handle* h, int32_t which);
 /* Size in elements of a field. */
 int64_t mflbm_field_size(const mflbm_handle* h, int32_t field);
 
+/* Plain device-to-device copy of `bytes` (two temporary buffers), repeated `repeats` times;
+ * returns the mean device time of one copy.  No counterpart in the reference: it calibrates
+ * bandwidth figures and the profiler's FETCH_SIZE / WRITE_SIZE counters on the same GPU. */
+int mflbm_bandwidth_probe(mflbm_handle* h, int64_t bytes, int32_t repeats, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

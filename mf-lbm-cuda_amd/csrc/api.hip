@@ -18,6 +18,7 @@
 #include "geometry.h"
 #include "lattice.h"
 #include "launch.h"
+#include "layout.h"
 
 using namespace mflbm;
 
@@ -27,16 +28,28 @@ struct mflbm_handle {
     int prec = MFLBM_FP64;
     int open_z = 0;
     hipStream_t stream = nullptr;
-    // fields (element type double or float)
+    // dense fields in the reference layout (element type double or float).  `pdf` is the
+    // transfer/background copy of the populations; the step works on the compact cells.
     void *pdf = nullptr, *phi = nullptr, *cnx = nullptr, *cny = nullptr, *cnz = nullptr, *cnorm = nullptr,
          *curv = nullptr, *snx = nullptr, *sny = nullptr, *snz = nullptr;
     uint8_t* walls = nullptr;
     int8_t* wtype = nullptr;
     uint32_t *l_fluid = nullptr, *l_fluid_ext = nullptr, *l_sb = nullptr;
     int n_fluid = 0, n_fluid_ext = 0, n_sb = 0;
+    // fluid-compact population cells
+    void* cells = nullptr;      // 38 * ns
+    int64_t ns = 0;
+    void* aux = nullptr;        // 4 * n_fluid force records
+    int32_t* nb = nullptr;      // 18 * n_fluid
+    int32_t* pairs[2] = {nullptr, nullptr};
+    int npairs[2] = {0, 0};
+    int32_t* zmap = nullptr;
+    int32_t *ext_node = nullptr, *ext_off = nullptr;
+    int n_ext = 0;
+    int32_t* zstart = nullptr;
     // monitor scratch
     double *mon_part = nullptr, *mon_out = nullptr;
-    int mon_rows = 0, mon_nsplit = 0;
+    int mon_nsplit = 0;
     // captured step graphs: [0] even step, [1] odd step
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     std::string err;
@@ -83,6 +96,18 @@ StepConst<T> consts(const mflbm_handle* h) {
     return c;
 }
 
+template <typename T>
+Sparse<T> sparse(const mflbm_handle* h) {
+    Sparse<T> S;
+    S.f = (T*)h->cells;
+    S.ns = h->ns;
+    S.nf = h->n_fluid;
+    S.nb = h->nb;
+    S.list = h->l_fluid;
+    S.aux = (T*)h->aux;
+    return S;
+}
+
 NodeLists lists(const mflbm_handle* h) {
     NodeLists L;
     L.fluid = h->l_fluid; L.n_fluid = h->n_fluid;
@@ -95,17 +120,21 @@ NodeLists lists(const mflbm_handle* h) {
 template <typename T>
 void enqueue_step(mflbm_handle* h, bool odd, hipEvent_t* ev) {
     const Fields<T> F = fields<T>(h);
+    const Sparse<T> S = sparse<T>(h);
     const StepConst<T> P = consts<T>(h);
     const NodeLists L = lists(h);
     hipStream_t s = h->stream;
     int e = 0;
     auto mark = [&]() { if (ev) (void)hipEventRecord(ev[e++], s); };
-    Launch<T>::collide(F, P, L, odd, s);
+    Launch<T>::collide(S, F.phi, h->d, P, odd, s);
     mark();
-    if (h->p.kper || h->p.jper) Launch<T>::periodic(F, h->p.kper, h->p.jper, odd, s);
+    if (h->p.kper || h->p.jper) {
+        Launch<T>::periodic_pdf(S, h->pairs[odd ? 1 : 0], h->npairs[odd ? 1 : 0], s);
+        Launch<T>::periodic_phi(F.phi, h->d, h->p.kper, h->p.jper, s);
+    }
     if (h->open_z) {
         const int in = h->p.inlet_bc == 2, out = h->p.outlet_bc == 2;
-        if (in || out) Launch<T>::zou_he(F, P, in, out, odd, s);
+        if (in || out) Launch<T>::zou_he(S, F, P, h->zmap, in, out, odd, s);
     }
     mark();
     Launch<T>::phi_extrap(F, L, s);
@@ -114,7 +143,7 @@ void enqueue_step(mflbm_handle* h, bool odd, hipEvent_t* ev) {
     mark();
     Launch<T>::normal_extrap(F, L, s);
     mark();
-    Launch<T>::curvature(F, L, s);
+    Launch<T>::curvature(F, P, L, (T*)h->aux, s);
     mark();
 }
 
@@ -126,7 +155,22 @@ void enqueue_gradient(mflbm_handle* h) {
     Launch<T>::phi_extrap(F, L, h->stream);
     Launch<T>::normals(F, P, L, h->stream);
     Launch<T>::normal_extrap(F, L, h->stream);
-    Launch<T>::curvature(F, L, h->stream);
+    Launch<T>::curvature(F, P, L, (T*)h->aux, h->stream);
+}
+
+// dense reference layout <-> compact cells
+void cells_transfer(mflbm_handle* h, bool to_dense) {
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::cells(sparse<double>(h), (double*)h->pdf, h->d, h->ext_node, h->ext_off, h->n_ext, to_dense, h->stream);
+    else
+        Launch<float>::cells(sparse<float>(h), (float*)h->pdf, h->d, h->ext_node, h->ext_off, h->n_ext, to_dense, h->stream);
+}
+
+void aux_refresh(mflbm_handle* h) {
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::aux_refresh(fields<double>(h), consts<double>(h), lists(h), (double*)h->aux, h->stream);
+    else
+        Launch<float>::aux_refresh(fields<float>(h), consts<float>(h), lists(h), (float*)h->aux, h->stream);
 }
 
 void step_dispatch(mflbm_handle* h, bool odd, hipEvent_t* ev) {
@@ -138,12 +182,15 @@ void free_all(mflbm_handle* h) {
     for (int g = 0; g < 2; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
-                    h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out};
+                    h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out,
+                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->ext_node, h->ext_off, h->zstart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
+    h->cells = h->aux = nullptr;
+    h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->ext_node = h->ext_off = h->zstart = nullptr;
     if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
 }
 
@@ -175,7 +222,7 @@ int build_graphs(mflbm_handle* h) {
         hipError_t e = hipStreamEndCapture(h->stream, &graph);
         if (e != hipSuccess) return fail(h, "graph capture failed: %s", hipGetErrorString(e));
         e = hipGraphInstantiate(&h->graph[g], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(h, "graph instantiate failed: %s", hipGetErrorString(e));
     }
     return 0;
@@ -184,6 +231,10 @@ int build_graphs(mflbm_handle* h) {
 void invalidate_graphs(mflbm_handle* h) {
     for (int g = 0; g < 2; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
+}
+
+__global__ __launch_bounds__(256) void k_probe_copy(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
 }
 
 }  // namespace
@@ -261,14 +312,14 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     }
     hipError_t ge = geometry_preprocess(d, (int)p.iper, (int)p.jper, (int)p.kper, geo_dev, h->walls, h->wtype, sx, sy, sz,
                                         h->stream);
-    hipFree(geo_dev);
+    (void)hipFree(geo_dev);
     if (ge != hipSuccess) return fail(h, "geometry preprocessing failed: %s", hipGetErrorString(ge));
     if (h->prec == MFLBM_FP32) {
         Launch<float>::to_real(sx, (float*)h->snx, d.S4, h->stream);
         Launch<float>::to_real(sy, (float*)h->sny, d.S4, h->stream);
         Launch<float>::to_real(sz, (float*)h->snz, d.S4, h->stream);
         HIPCHK(h, hipStreamSynchronize(h->stream));
-        hipFree(sx); hipFree(sy); hipFree(sz);
+        (void)hipFree(sx); (void)hipFree(sy); (void)hipFree(sz);
     }
 
     // node lists from the classified geometry
@@ -297,9 +348,34 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     HIPCHK(h, upload_list(&h->l_fluid_ext, lfe));
     HIPCHK(h, upload_list(&h->l_sb, lsb));
 
-    // monitor scratch: blocks of ~8 rows per slice
-    h->mon_rows = std::max(1, std::min(d.ny, 8));
-    h->mon_nsplit = (d.ny + h->mon_rows - 1) / h->mon_rows;
+    // fluid-compact population cells
+    LayoutHost lay;
+    std::string lerr;
+    if (!build_layout(d, lf, (int)p.kper, (int)p.jper, h->open_z, p.inlet_bc == 2, p.outlet_bc == 2, lay, lerr))
+        return fail(h, "layout: %s", lerr.c_str());
+    h->ns = lay.ns;
+    h->n_ext = lay.ext_off[19];
+    HIPCHK(h, hipMalloc(&h->cells, 38 * (size_t)h->ns * es));
+    HIPCHK(h, hipMemsetAsync(h->cells, 0, 38 * (size_t)h->ns * es, h->stream));
+    HIPCHK(h, hipMalloc(&h->aux, std::max<size_t>(1, (size_t)4 * h->n_fluid) * es));
+    HIPCHK(h, hipMemsetAsync(h->aux, 0, std::max<size_t>(1, (size_t)4 * h->n_fluid) * es, h->stream));
+    auto upload_i32 = [&](int32_t** dst, const std::vector<int32_t>& v) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(int32_t));
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    };
+    HIPCHK(h, upload_i32(&h->nb, lay.nb));
+    HIPCHK(h, upload_i32(&h->ext_node, lay.ext_node));
+    HIPCHK(h, upload_i32(&h->ext_off, lay.ext_off));
+    HIPCHK(h, upload_i32(&h->pairs[0], lay.pairs[0]));
+    HIPCHK(h, upload_i32(&h->pairs[1], lay.pairs[1]));
+    h->npairs[0] = (int)(lay.pairs[0].size() / 3);
+    h->npairs[1] = (int)(lay.pairs[1].size() / 3);
+    HIPCHK(h, upload_i32(&h->zmap, lay.zmap));
+    HIPCHK(h, upload_i32(&h->zstart, lay.zstart));
+
+    // monitor scratch: every z-slice reduced by mon_nsplit blocks
+    h->mon_nsplit = 16;
     HIPCHK(h, hipMalloc(&h->mon_part, (size_t)d.nz * h->mon_nsplit * 10 * sizeof(double)));
     HIPCHK(h, hipMalloc(&h->mon_out, (size_t)d.nz * 10 * sizeof(double)));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -335,6 +411,7 @@ int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, 
         }
     if (h->prec == MFLBM_FP64) Launch<double>::init_eq(fields<double>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
     else Launch<float>::init_eq(fields<float>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    cells_transfer(h, false);
     hipError_t e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(h->stream);
     for (int a = 0; a < 4; ++a)
@@ -400,6 +477,10 @@ int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes) {
     int64_t n; size_t es; void* ptr;
     if (field_info(h, field, &n, &es, &ptr)) return fail(h, "unknown field %d", field);
     if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
+    if (field == MFLBM_PDF) {  // scatter the compact cells over the dense copy first
+        cells_transfer(h, true);
+        HIPCHK(h, hipGetLastError());
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(dst, ptr, bytes, hipMemcpyDeviceToHost));
     return 0;
@@ -413,6 +494,11 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
     if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
+    if (field == MFLBM_PDF) cells_transfer(h, false);
+    if (field == MFLBM_CN_X || field == MFLBM_CN_Y || field == MFLBM_CN_Z || field == MFLBM_C_NORM || field == MFLBM_CURV)
+        aux_refresh(h);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -426,8 +512,8 @@ int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w) {
             HIPCHK(h, hipMalloc(&dv[a], n * sizeof(double)));
             HIPCHK(h, hipMemsetAsync(dv[a], 0, n * sizeof(double), h->stream));
         }
-    if (h->prec == MFLBM_FP64) Launch<double>::macro(fields<double>(h), consts<double>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
-    else Launch<float>::macro(fields<float>(h), consts<float>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    if (h->prec == MFLBM_FP64) Launch<double>::macro(sparse<double>(h), consts<double>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
+    else Launch<float>::macro(sparse<float>(h), consts<float>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     for (int a = 0; a < 4; ++a)
@@ -443,9 +529,11 @@ int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out) {
     if (!h || !out) return -1;
     const int nz = h->d.nz;
     if (h->prec == MFLBM_FP64)
-        Launch<double>::monitor(fields<double>(h), consts<double>(h), h->mon_rows, h->mon_nsplit, h->mon_part, h->mon_out, h->stream);
+        Launch<double>::monitor(sparse<double>(h), consts<double>(h), (const double*)h->phi, h->d, h->zstart, h->mon_nsplit,
+                                h->mon_part, h->mon_out, h->stream);
     else
-        Launch<float>::monitor(fields<float>(h), consts<float>(h), h->mon_rows, h->mon_nsplit, h->mon_part, h->mon_out, h->stream);
+        Launch<float>::monitor(sparse<float>(h), consts<float>(h), (const float*)h->phi, h->d, h->zstart, h->mon_nsplit,
+                               h->mon_part, h->mon_out, h->stream);
     HIPCHK(h, hipGetLastError());
     std::vector<double> res((size_t)10 * nz);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -473,6 +561,34 @@ int mflbm_sync(mflbm_handle* h) {
 int64_t mflbm_count(const mflbm_handle* h, int32_t which) {
     if (!h) return -1;
     return which == 0 ? h->n_fluid : which == 1 ? h->n_fluid_ext : which == 2 ? h->n_sb : -1;
+}
+
+int mflbm_bandwidth_probe(mflbm_handle* h, int64_t bytes, int32_t repeats, double* ms_per_copy) {
+    if (!h || !ms_per_copy || bytes <= 0 || repeats <= 0) return -1;
+    const int64_t n = bytes / (int64_t)sizeof(double4);
+    double4 *a = nullptr, *b = nullptr;
+    HIPCHK(h, hipMalloc(&a, n * sizeof(double4)));
+    hipError_t e = hipMalloc(&b, n * sizeof(double4));
+    if (e != hipSuccess) { (void)hipFree(a); return fail(h, "probe alloc: %s", hipGetErrorString(e)); }
+    (void)hipMemsetAsync(a, 0, n * sizeof(double4), h->stream);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned grid = 256 * 8 * 4;
+    k_probe_copy<<<grid, 256, 0, h->stream>>>(a, b, n);  // warm-up
+    (void)hipEventRecord(e0, h->stream);
+    for (int r = 0; r < repeats; ++r) k_probe_copy<<<grid, 256, 0, h->stream>>>(a, b, n);
+    (void)hipEventRecord(e1, h->stream);
+    e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(h, "probe: %s", hipGetErrorString(e));
+    *ms_per_copy = (double)ms / repeats;
+    return 0;
 }
 
 int64_t mflbm_field_size(const mflbm_handle* h, int32_t field) {

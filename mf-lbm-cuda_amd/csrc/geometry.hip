@@ -192,9 +192,9 @@ hipError_t geometry_preprocess(const Dims& d, int iper, int jper, int kper, cons
     double *a = nullptr, *b = nullptr;
     hipError_t e;
     if ((e = hipMalloc(&wt, t.N)) != hipSuccess) return e;
-    if ((e = hipMalloc(&wc, t.N)) != hipSuccess) { hipFree(wt); return e; }
-    if ((e = hipMalloc(&a, t.N * sizeof(double))) != hipSuccess) { hipFree(wt); hipFree(wc); return e; }
-    if ((e = hipMalloc(&b, t.N * sizeof(double))) != hipSuccess) { hipFree(wt); hipFree(wc); hipFree(a); return e; }
+    if ((e = hipMalloc(&wc, t.N)) != hipSuccess) { (void)hipFree(wt); return e; }
+    if ((e = hipMalloc(&a, t.N * sizeof(double))) != hipSuccess) { (void)hipFree(wt); (void)hipFree(wc); return e; }
+    if ((e = hipMalloc(&b, t.N * sizeof(double))) != hipSuccess) { (void)hipFree(wt); (void)hipFree(wc); (void)hipFree(a); return e; }
     const int B = 256;
     k_geo_fill<<<nblk(t.N, B), B, 0, s>>>(geo_dev, wt, t, iper, jper, kper);
     k_geo_classify<<<nblk(t.N, B), B, 0, s>>>(wt, wc, t);
@@ -208,7 +208,7 @@ hipError_t geometry_preprocess(const Dims& d, int iper, int jper, int kper, cons
     k_geo_extract<<<nblk(d.S4 > d.S2 ? d.S4 : d.S2, B), B, 0, s>>>(wt, wc, t, d, walls, wtype);
     e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(s);
-    hipFree(wt); hipFree(wc); hipFree(a); hipFree(b);
+    (void)hipFree(wt); (void)hipFree(wc); (void)hipFree(a); (void)hipFree(b);
     return e != hipSuccess ? e : e2;
 }
 

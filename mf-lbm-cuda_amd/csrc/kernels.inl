@@ -5,7 +5,8 @@
 //
 // Reference correspondence (src/main_iteration_GPU.cu):
 //   k_collide<ODD>     kernel_odd_color_GPU / kernel_even_color_GPU   (56-388 / 395-726)
-//   k_periodic<ODD>    perioidc_BC_{Z,Y,ZY}_{odd,even}, periodic_phi_{Z,Y,ZY} (1529-1733)
+//   k_periodic_pdf     perioidc_BC_{Z,Y,ZY}_{odd,even}                (1529-1689)
+//   k_periodic_phi     periodic_phi_{Z,Y,ZY}                          (1691-1733)
 //   k_zou_he<AFTER>    inlet/outlet_Zou_He_pressure_BC_{before,after}_odd_GPU (1085-1524)
 //   k_phi_extrap       extrapolate_phi_toSolid                        (732-755)
 //   k_normals          normalDirectionsOfInterfaces + alter_color_gradient_solid_surface_GPU
@@ -20,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include "lattice.h"
+#include "zouhe.h"
 
 namespace mflbm {
 namespace dev {
@@ -48,7 +50,7 @@ __device__ __forceinline__ T wq(int q) { return q == 0 ? W<T>::w0 : (q <= 6 ? W<
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
-                                          T cnz, T curv, T cnorm, T (&o1)[19], T (&o2)[19]) {
+                                          T cnz, T ftmp, T (&o1)[19], T (&o2)[19]) {
     T ft[19];
 #pragma unroll
     for (int q = 0; q < 19; ++q) ft[q] = g1[q] + g2[q];
@@ -57,7 +59,8 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     for (int q = 1; q < 19; ++q) { rho1 = rho1 + g1[q]; rho2 = rho2 + g2[q]; }
     const T phi = (rho1 - rho2) / (rho1 + rho2);
 
-    T tmp = T(0.5) * P.gamma * curv * cnorm;
+    // ftmp = 0.5 * gamma * curv * c_norm, evaluated in that order by k_curvature
+    T tmp = ftmp;
     const T fx = tmp * cnx, fy = tmp * cny, fz = tmp * cnz + P.force_z;
 
     const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
@@ -182,79 +185,78 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     return phi;
 }
 
+
 // ---------------------------------------------------------------------------------------
-// AA-pattern collide/stream over the interior fluid-node list.
-// ODD:  pull q from (x - e_q).slot q, push q to (x + e_q).slot opp(q)
-// EVEN: read q from x.slot opp(q), write q to x.slot q
-// Each (node, slot) is touched by exactly one node per sweep, so the update is in place.
+// Fluid-compact population storage.
+// The 38 slot arrays hold, per slot, the interior fluid nodes first (index = position in
+// the fluid list) followed by the slot's extra cells: every non-fluid (node, slot) pair a
+// step ever reads or writes (bounce-back buffers in solid neighbours, ghost and periodic
+// image cells, cells read by the Zou-He boundaries).  nb[(q-1)*nf + t] is the index of
+// cell (x_t + e_q, slot opp(q)) in slot array opp(q): an odd step pushes direction q into
+// that cell and pulls direction opp(q) out of it (the same cell, hence in place).  Fluid
+// neighbours have the same index in every slot array (their list position).
 // ---------------------------------------------------------------------------------------
+
+// AA-pattern collide/stream of one interior fluid node (src/main_iteration_GPU.cu:56-388
+// odd, 395-726 even).  Odd: pull q from (x - e_q).slot q, push q to (x + e_q).slot opp(q);
+// even: read q from x.slot opp(q), write q to x.slot q.  Each cell is read and written by
+// exactly one node per sweep, so the update is in place.
 template <typename T, bool ODD>
-__global__ __launch_bounds__(256) void k_collide(Fields<T> F, StepConst<T> P, const uint32_t* __restrict__ list,
-                                                 int n) {
+__global__ __launch_bounds__(256) void k_collide(Sparse<T> S, T* __restrict__ phi, Dims d, StepConst<T> P) {
     const int t = xcd_block() * 256 + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    unpack_ijk(list[t], i, j, k);
-    const Dims& d = F.d;
-    const int c = i1(d, i, j, k);
-    const int64_t S1 = d.S1;
+    if (t >= S.nf) return;
+    const int64_t NS = S.ns;
     T g1[19], g2[19], o1[19], o2[19];
+    int nb[19];
+    if (ODD) {
+        nb[0] = t;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-        const int src = ODD ? c - (EX[q] + d.X1 * (EY[q] + d.Y1 * EZ[q])) : c;
-        const int slot = ODD ? q : OPP[q];
-        g1[q] = F.pdf[src + S1 * slot];
-        g2[q] = F.pdf[src + S1 * (slot + 19)];
+        for (int q = 1; q < 19; ++q) nb[q] = S.nb[(int64_t)(q - 1) * S.nf + t];
     }
-    const int c2 = i2(d, i, j, k);
-    const T ph = collide_node<T>(P, g1, g2, F.cnx[c2], F.cny[c2], F.cnz[c2], F.curv[c], F.cnorm[c2], o1, o2);
-    F.phi[i4(d, i, j, k)] = ph;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-        const int dst = ODD ? c + (EX[q] + d.X1 * (EY[q] + d.Y1 * EZ[q])) : c;
+        const int64_t src = ODD ? (int64_t)nb[OPP[q]] : (int64_t)t;
+        const int slot = ODD ? q : OPP[q];
+        g1[q] = S.f[slot * NS + src];
+        g2[q] = S.f[(slot + 19) * NS + src];
+    }
+    const T* a = S.aux + 4 * (int64_t)t;
+    const T ph = collide_node<T>(P, g1, g2, a[0], a[1], a[2], a[3], o1, o2);
+    int i, j, k;
+    unpack_ijk(S.list[t], i, j, k);
+    phi[i4(d, i, j, k)] = ph;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const int64_t dst = ODD ? (int64_t)nb[q] : (int64_t)t;
         const int slot = ODD ? OPP[q] : q;
-        F.pdf[dst + S1 * slot] = o1[q];
-        F.pdf[dst + S1 * (slot + 19)] = o2[q];
+        S.f[slot * NS + dst] = o1[q];
+        S.f[(slot + 19) * NS + dst] = o2[q];
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// Periodic ghost copies of populations and phi, all enabled axes in one launch.  The
-// reference runs the Z, Y and ZY kernels in sequence.  After an odd step the Z-face, Y-face
-// and edge copies all write four edge slots ((j,k) = (1,1) slot 18, (1,nz) slot 16,
-// (ny,1) slot 17, (ny,nz) slot 15); the edge copy runs last and wins.  Here the face
-// threads leave those slots to the edge threads, which makes every remaining write
-// unique, and no thread reads a slot another thread writes, so one launch reproduces the
-// sequence.  Thread ranges: [0, nx*ny) z faces, [.., +nx*nz) y faces, [.., +nx) y-z edges.
-// ---------------------------------------------------------------------------------------
-template <typename T, bool ODD>
-__global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jper) {
-    const Dims& d = F.d;
+// Periodic population copies, precomputed as (slot, destination cell, source cell) in the
+// reference's kernel order with repeated destinations resolved to the last writer
+// (perioidc_BC_{Z,Y,ZY}_{odd,even}, src/main_iteration_GPU.cu:1529-1689).
+template <typename T>
+__global__ __launch_bounds__(256) void k_periodic_pdf(Sparse<T> S, const int32_t* __restrict__ pairs, int n) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int slot = pairs[3 * t], dst = pairs[3 * t + 1], src = pairs[3 * t + 2];
+    const int64_t NS = S.ns;
+    S.f[slot * NS + dst] = S.f[slot * NS + src];
+    S.f[(slot + 19) * NS + dst] = S.f[(slot + 19) * NS + src];
+}
+
+// Periodic ghost layers of phi (periodic_phi_{Z,Y,ZY}, src/main_iteration_GPU.cu:1691-1733);
+// the three groups read interior cells and write disjoint ghost cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_periodic_phi(T* __restrict__ phi, Dims d, int kper, int jper) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     int t = blockIdx.x * 256 + threadIdx.x;
-    const int64_t S1 = d.S1;
-    auto P = [&](int a, int b, int c) -> T& { return F.phi[i4(d, a, b, c)]; };
-    auto cp = [&](int dst, int src, int slot) {
-        F.pdf[dst + S1 * slot] = F.pdf[src + S1 * slot];
-        F.pdf[dst + S1 * (slot + 19)] = F.pdf[src + S1 * (slot + 19)];
-    };
+    auto P = [&](int a, int b, int c) -> T& { return phi[i4(d, a, b, c)]; };
     const int nzf = kper ? nx * ny : 0, nyf = jper ? nx * nz : 0, ne = (kper && jper) ? nx : 0;
-    const bool edges = kper && jper;
     if (t < nzf) {
         const int i = t % nx + 1, j = t / nx + 1;
-        const int up[5] = {6, 14, 13, 18, 17}, dn[5] = {5, 11, 12, 15, 16};
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            if (!ODD) {
-                cp(i1(d, i, j, nz + 1), i1(d, i, j, 1), up[m]);
-                cp(i1(d, i, j, 0), i1(d, i, j, nz), dn[m]);
-            } else {
-                const bool edge_up = edges && ((j == 1 && up[m] == 18) || (j == ny && up[m] == 17));
-                const bool edge_dn = edges && ((j == 1 && dn[m] == 16) || (j == ny && dn[m] == 15));
-                if (!edge_up) cp(i1(d, i, j, 1), i1(d, i, j, nz + 1), up[m]);
-                if (!edge_dn) cp(i1(d, i, j, nz), i1(d, i, j, 0), dn[m]);
-            }
-        }
 #pragma unroll
         for (int kk = 1; kk <= 4; ++kk) {
             P(i, j, kk + nz) = P(i, j, kk);
@@ -265,19 +267,6 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
     t -= nzf;
     if (t < nyf) {
         const int i = t % nx + 1, k = t / nx + 1;
-        const int up[5] = {4, 10, 9, 16, 18}, dn[5] = {3, 7, 8, 15, 17};
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            if (!ODD) {
-                cp(i1(d, i, ny + 1, k), i1(d, i, 1, k), up[m]);
-                cp(i1(d, i, 0, k), i1(d, i, ny, k), dn[m]);
-            } else {
-                const bool edge_up = edges && ((k == 1 && up[m] == 18) || (k == nz && up[m] == 16));
-                const bool edge_dn = edges && ((k == 1 && dn[m] == 17) || (k == nz && dn[m] == 15));
-                if (!edge_up) cp(i1(d, i, 1, k), i1(d, i, ny + 1, k), up[m]);
-                if (!edge_dn) cp(i1(d, i, ny, k), i1(d, i, 0, k), dn[m]);
-            }
-        }
 #pragma unroll
         for (int jj = 1; jj <= 4; ++jj) {
             P(i, jj + ny, k) = P(i, jj, k);
@@ -288,17 +277,6 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
     t -= nyf;
     if (t < ne) {
         const int i = t + 1;
-        if (!ODD) {
-            cp(i1(d, i, ny + 1, nz + 1), i1(d, i, 1, 1), 18);
-            cp(i1(d, i, ny + 1, 0), i1(d, i, 1, nz), 16);
-            cp(i1(d, i, 0, nz + 1), i1(d, i, ny, 1), 17);
-            cp(i1(d, i, 0, 0), i1(d, i, ny, nz), 15);
-        } else {
-            cp(i1(d, i, 1, 1), i1(d, i, ny + 1, nz + 1), 18);
-            cp(i1(d, i, 1, nz), i1(d, i, ny + 1, 0), 16);
-            cp(i1(d, i, ny, 1), i1(d, i, 0, nz + 1), 17);
-            cp(i1(d, i, ny, nz), i1(d, i, 0, 0), 15);
-        }
         for (int kk = 1; kk <= 4; ++kk)
             for (int jj = 1; jj <= 4; ++jj) {
                 P(i, jj - 4, kk - 4) = P(i, ny + jj - 4, nz + kk - 4);
@@ -309,125 +287,56 @@ __global__ __launch_bounds__(256) void k_periodic(Fields<T> F, int kper, int jpe
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// Zou-He pressure inlet (k = 1) and outlet (k = nz), one thread per (i, j).  The inlet
-// writes plane k <= 1 and the outlet plane k >= nz, so one launch serves both (nz >= 3).
-// Sums follow the slot order of each reference variant (the orders differ between the
-// before-odd and after-odd kernels and so does the rounding).
-// ---------------------------------------------------------------------------------------
+// Zou-He inlet/outlet, one thread per (i, j).  zmap[plane][slot][b * (nx+2) + a] gives the
+// cell of (a, b, k) for the six planes k = 0, 1, 2, nz-1, nz, nz+1.
 template <typename T, bool AFTER_ODD>
-__global__ __launch_bounds__(256) void k_zou_he(Fields<T> F, StepConst<T> P, int do_in, int do_out) {
+__global__ __launch_bounds__(256) void k_zou_he(Sparse<T> S, Fields<T> F, StepConst<T> P, const int32_t* __restrict__ zmap,
+                                                int do_in, int do_out) {
     const Dims& d = F.d;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= d.nx * d.ny) return;
     const int i = t % d.nx + 1, j = t / d.nx + 1, nz = d.nz;
-    const int64_t S1 = d.S1;
-    const T c3 = T(0.333333333333333333), c6 = T(0.166666666666666667);
-    auto f = [&](int a, int b, int c, int e, int g) -> T& { return F.pdf[i1(d, a, b, c) + S1 * (e + 19 * g)]; };
-    if (do_in) {
-        const int w = F.walls[i2(d, i, j, 1)];
-        T& p0 = F.phi[i4(d, i, j, 0)];
-        p0 = P.phi_inlet * (1 - w) + p0 * w;
-        const T pv = p0;
-        F.phi[i4(d, i, j, -1)] = pv;
-        F.phi[i4(d, i, j, -2)] = pv;
-        F.phi[i4(d, i, j, -3)] = pv;
-        T rT[2];
-        rT[1] = P.rho_in;
-        rT[0] = P.rho_in * P.sa_inject;
-        rT[1] = rT[1] - rT[0];
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-            if (!AFTER_ODD) {
-                T tt = (rT[g] - (f(i, j, 1, 0, g) + f(i - 1, j, 1, 1, g) + f(i + 1, j, 1, 2, g) + f(i, j - 1, 1, 3, g) +
-                                 f(i, j + 1, 1, 4, g) + f(i - 1, j - 1, 1, 7, g) + f(i + 1, j - 1, 1, 8, g) +
-                                 f(i - 1, j + 1, 1, 9, g) + f(i + 1, j + 1, 1, 10, g) +
-                                 T(2.0) * (f(i, j, 2, 6, g) + f(i + 1, j, 2, 14, g) + f(i - 1, j, 2, 13, g) +
-                                           f(i, j + 1, 2, 18, g) + f(i, j - 1, 2, 17, g)))) * P.relaxation;
-                T tnx = T(0.5) * (f(i - 1, j, 1, 1, g) + f(i - 1, j - 1, 1, 7, g) + f(i - 1, j + 1, 1, 9, g) -
-                                  (f(i + 1, j, 1, 2, g) + f(i + 1, j - 1, 1, 8, g) + f(i + 1, j + 1, 1, 10, g)));
-                T tny = T(0.5) * (f(i, j - 1, 1, 3, g) + f(i - 1, j - 1, 1, 7, g) + f(i + 1, j - 1, 1, 8, g) -
-                                  (f(i, j + 1, 1, 4, g) + f(i + 1, j + 1, 1, 10, g) + f(i - 1, j + 1, 1, 9, g)));
-                f(i, j, 0, 5, g) = (f(i, j, 2, 6, g) + c3 * tt) * (1 - w) + f(i, j, 0, 5, g) * w;
-                f(i - 1, j, 0, 11, g) = (f(i + 1, j, 2, 14, g) + c6 * tt - tnx) * (1 - w) + f(i - 1, j, 0, 11, g) * w;
-                f(i + 1, j, 0, 12, g) = (f(i - 1, j, 2, 13, g) + c6 * tt + tnx) * (1 - w) + f(i + 1, j, 0, 12, g) * w;
-                f(i, j - 1, 0, 15, g) = (f(i, j + 1, 2, 18, g) + c6 * tt - tny) * (1 - w) + f(i, j - 1, 0, 15, g) * w;
-                f(i, j + 1, 0, 16, g) = (f(i, j - 1, 2, 17, g) + c6 * tt + tny) * (1 - w) + f(i, j + 1, 0, 16, g) * w;
-            } else {
-                T tt = (rT[g] - (f(i, j, 1, 0, g) + f(i, j, 1, 2, g) + f(i, j, 1, 1, g) + f(i, j, 1, 4, g) +
-                                 f(i, j, 1, 3, g) + f(i, j, 1, 8, g) + f(i, j, 1, 7, g) + f(i, j, 1, 10, g) +
-                                 f(i, j, 1, 9, g) +
-                                 T(2.0) * (f(i, j, 1, 5, g) + f(i, j, 1, 11, g) + f(i, j, 1, 12, g) +
-                                           f(i, j, 1, 15, g) + f(i, j, 1, 16, g)))) * P.relaxation;
-                T tnx = T(0.5) * (f(i, j, 1, 2, g) + f(i, j, 1, 8, g) + f(i, j, 1, 10, g) -
-                                  (f(i, j, 1, 1, g) + f(i, j, 1, 7, g) + f(i, j, 1, 9, g)));
-                T tny = T(0.5) * (f(i, j, 1, 4, g) + f(i, j, 1, 9, g) + f(i, j, 1, 10, g) -
-                                  (f(i, j, 1, 3, g) + f(i, j, 1, 8, g) + f(i, j, 1, 7, g)));
-                f(i, j, 1, 6, g) = (f(i, j, 1, 5, g) + c3 * tt) * (1 - w) + f(i, j, 1, 6, g) * w;
-                f(i, j, 1, 13, g) = (f(i, j, 1, 12, g) + c6 * tt + tnx) * (1 - w) + f(i, j, 1, 13, g) * w;
-                f(i, j, 1, 14, g) = (f(i, j, 1, 11, g) + c6 * tt - tnx) * (1 - w) + f(i, j, 1, 14, g) * w;
-                f(i, j, 1, 17, g) = (f(i, j, 1, 16, g) + c6 * tt + tny) * (1 - w) + f(i, j, 1, 17, g) * w;
-                f(i, j, 1, 18, g) = (f(i, j, 1, 15, g) + c6 * tt - tny) * (1 - w) + f(i, j, 1, 18, g) * w;
-            }
-        }
+    const int64_t NS = S.ns;
+    const int64_t plane = (int64_t)19 * d.X1 * d.Y1;
+    auto f = [&](int a, int b, int c, int e, int g) -> T& {
+        const int pl = c <= 2 ? c : 3 + (c - (nz - 1));
+        const int cell = zmap[pl * plane + (int64_t)e * d.X1 * d.Y1 + b * d.X1 + a];
+        return S.f[(e + 19 * g) * NS + cell];
+    };
+    auto wall = [&](int a, int b, int c) -> int { return F.walls[i2(d, a, b, c)]; };
+    auto ph = [&](int a, int b, int c) -> T& { return F.phi[i4(d, a, b, c)]; };
+    zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
+}
+
+// Dense reference layout <-> compact storage.  Fluid cells: t < nf; extra cells: the
+// ext_node list of each slot (offsets ext_off[0..19]).
+template <typename T, bool TO_DENSE>
+__global__ __launch_bounds__(256) void k_fluid_cells(Sparse<T> S, T* __restrict__ dense, Dims d) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= S.nf) return;
+    int i, j, k;
+    unpack_ijk(S.list[t], i, j, k);
+    const int64_t c = i1(d, i, j, k);
+    for (int s = 0; s < 38; ++s) {
+        if (TO_DENSE) dense[s * d.S1 + c] = S.f[s * S.ns + t];
+        else S.f[s * S.ns + t] = dense[s * d.S1 + c];
     }
-    if (do_out) {
-        const int w = F.walls[i2(d, i, j, nz)];
-        const T ph = F.phi[i4(d, i, j, nz)];
-#pragma unroll
-        for (int m = 1; m <= 4; ++m) F.phi[i4(d, i, j, nz + m)] = ph;
-        T t1;
-        if (!AFTER_ODD) {
-            t1 = f(i, j, nz, 0, 0) + f(i - 1, j, nz, 1, 0) + f(i + 1, j, nz, 2, 0) + f(i, j - 1, nz, 3, 0) +
-                 f(i, j + 1, nz, 4, 0) + f(i - 1, j - 1, nz, 7, 0) + f(i + 1, j - 1, nz, 8, 0) +
-                 f(i - 1, j + 1, nz, 9, 0) + f(i + 1, j + 1, nz, 10, 0) +
-                 T(2.0) * (f(i, j, nz - 1, 5, 0) + f(i - 1, j, nz - 1, 11, 0) + f(i + 1, j, nz - 1, 12, 0) +
-                           f(i, j - 1, nz - 1, 15, 0) + f(i, j + 1, nz - 1, 16, 0)) +
-                 f(i, j, nz, 0, 1) + f(i - 1, j, nz, 1, 1) + f(i + 1, j, nz, 2, 1) + f(i, j - 1, nz, 3, 1) +
-                 f(i, j + 1, nz, 4, 1) + f(i - 1, j - 1, nz, 7, 1) + f(i + 1, j - 1, nz, 8, 1) +
-                 f(i - 1, j + 1, nz, 9, 1) + f(i + 1, j + 1, nz, 10, 1) +
-                 T(2.0) * (f(i, j, nz - 1, 5, 1) + f(i - 1, j, nz - 1, 11, 1) + f(i + 1, j, nz - 1, 12, 1) +
-                           f(i, j - 1, nz - 1, 15, 1) + f(i, j + 1, nz - 1, 16, 1)) -
-                 P.rho_out;
-        } else {
-            t1 = f(i, j, nz, 0, 0) + f(i, j, nz, 2, 0) + f(i, j, nz, 1, 0) + f(i, j, nz, 4, 0) + f(i, j, nz, 3, 0) +
-                 f(i, j, nz, 8, 0) + f(i, j, nz, 7, 0) + f(i, j, nz, 10, 0) + f(i, j, nz, 9, 0) +
-                 T(2.0) * (f(i, j, nz, 6, 0) + f(i, j, nz, 14, 0) + f(i, j, nz, 13, 0) + f(i, j, nz, 18, 0) +
-                           f(i, j, nz, 17, 0)) +
-                 f(i, j, nz, 0, 1) + f(i, j, nz, 2, 1) + f(i, j, nz, 1, 1) + f(i, j, nz, 4, 1) + f(i, j, nz, 3, 1) +
-                 f(i, j, nz, 8, 1) + f(i, j, nz, 7, 1) + f(i, j, nz, 10, 1) + f(i, j, nz, 9, 1) +
-                 T(2.0) * (f(i, j, nz, 6, 1) + f(i, j, nz, 14, 1) + f(i, j, nz, 13, 1) + f(i, j, nz, 18, 1) +
-                           f(i, j, nz, 17, 1)) -
-                 P.rho_out;
-        }
-        T t2 = t1 * T(0.5) * (T(1.0) - ph);
-        t1 = t1 - t2;
-        const T tt2[2] = {t1, t2};
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-            const T tt = tt2[g];
-            if (!AFTER_ODD) {
-                T tnx = T(0.5) * (f(i - 1, j, nz, 1, g) + f(i - 1, j - 1, nz, 7, g) + f(i - 1, j + 1, nz, 9, g) -
-                                  (f(i + 1, j, nz, 2, g) + f(i + 1, j - 1, nz, 8, g) + f(i + 1, j + 1, nz, 10, g)));
-                T tny = T(0.5) * (f(i, j - 1, nz, 3, g) + f(i - 1, j - 1, nz, 7, g) + f(i + 1, j - 1, nz, 8, g) -
-                                  (f(i, j + 1, nz, 4, g) + f(i + 1, j + 1, nz, 10, g) + f(i - 1, j + 1, nz, 9, g)));
-                f(i, j, nz + 1, 6, g) = (f(i, j, nz - 1, 5, g) - c3 * tt) * (1 - w) + f(i, j, nz + 1, 6, g) * w;
-                f(i - 1, j, nz + 1, 13, g) = (f(i + 1, j, nz - 1, 12, g) - c6 * tt - tnx) * (1 - w) + f(i - 1, j, nz + 1, 13, g) * w;
-                f(i + 1, j, nz + 1, 14, g) = (f(i - 1, j, nz - 1, 11, g) - c6 * tt + tnx) * (1 - w) + f(i + 1, j, nz + 1, 14, g) * w;
-                f(i, j - 1, nz + 1, 17, g) = (f(i, j + 1, nz - 1, 16, g) - c6 * tt - tny) * (1 - w) + f(i, j - 1, nz + 1, 17, g) * w;
-                f(i, j + 1, nz + 1, 18, g) = (f(i, j - 1, nz - 1, 15, g) - c6 * tt + tny) * (1 - w) + f(i, j + 1, nz + 1, 18, g) * w;
-            } else {
-                T tnx = T(0.5) * (f(i, j, nz, 2, g) + f(i, j, nz, 8, g) + f(i, j, nz, 10, g) -
-                                  (f(i, j, nz, 1, g) + f(i, j, nz, 7, g) + f(i, j, nz, 9, g)));
-                T tny = T(0.5) * (f(i, j, nz, 4, g) + f(i, j, nz, 10, g) + f(i, j, nz, 9, g) -
-                                  (f(i, j, nz, 3, g) + f(i, j, nz, 7, g) + f(i, j, nz, 8, g)));
-                f(i, j, nz, 5, g) = (f(i, j, nz, 6, g) - c3 * tt) * (1 - w) + f(i, j, nz, 5, g) * w;
-                f(i, j, nz, 11, g) = (f(i, j, nz, 14, g) - c6 * tt + tnx) * (1 - w) + f(i, j, nz, 11, g) * w;
-                f(i, j, nz, 12, g) = (f(i, j, nz, 13, g) - c6 * tt - tnx) * (1 - w) + f(i, j, nz, 12, g) * w;
-                f(i, j, nz, 15, g) = (f(i, j, nz, 18, g) - c6 * tt + tny) * (1 - w) + f(i, j, nz, 15, g) * w;
-                f(i, j, nz, 16, g) = (f(i, j, nz, 17, g) - c6 * tt - tny) * (1 - w) + f(i, j, nz, 16, g) * w;
-            }
-        }
+}
+
+template <typename T, bool TO_DENSE>
+__global__ __launch_bounds__(256) void k_extra_cells(Sparse<T> S, T* __restrict__ dense, Dims d,
+                                                     const int32_t* __restrict__ ext_node,
+                                                     const int32_t* __restrict__ ext_off) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ext_off[19]) return;
+    int s = 0;
+    while (t >= ext_off[s + 1]) ++s;
+    const int e = t - ext_off[s];
+    const int64_t c = ext_node[t];
+    for (int g = 0; g < 2; ++g) {
+        const int sl = s + 19 * g;
+        if (TO_DENSE) dense[sl * d.S1 + c] = S.f[sl * S.ns + S.nf + e];
+        else S.f[sl * S.ns + S.nf + e] = dense[sl * d.S1 + c];
     }
 }
 
@@ -559,15 +468,12 @@ __global__ __launch_bounds__(256) void k_normal_extrap(Fields<T> F, const uint32
     F.cnx[c2] = sx / ws; F.cny[c2] = sy / ws; F.cnz[c2] = sz / ws;
 }
 
-// K7: curvature from the ISO4 derivatives of the normals, at interior fluid nodes.
+// K7: curvature from the ISO4 derivatives of the normals, at interior fluid nodes.  Also
+// writes the node's record for the next collide, in fluid-list order:
+// (cn_x, cn_y, cn_z, 0.5 * gamma * curv * c_norm).
 template <typename T>
-__global__ __launch_bounds__(256) void k_curvature(Fields<T> F, const uint32_t* __restrict__ list, int n) {
-    const int t = xcd_block() * 256 + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    unpack_ijk(list[t], i, j, k);
+__device__ __forceinline__ T curvature_at(const Fields<T>& F, int c2) {
     const Dims& d = F.d;
-    const int c2 = i2(d, i, j, k);
     const int sx = 1, sy = d.X2, sz = d.X2 * d.Y2;
     const T kxx = iso4<T, 0>(F.cnx, c2, sx, sy, sz);
     const T kyy = iso4<T, 1>(F.cny, c2, sx, sy, sz);
@@ -579,8 +485,41 @@ __global__ __launch_bounds__(256) void k_curvature(Fields<T> F, const uint32_t* 
     const T kzx = iso4<T, 0>(F.cnz, c2, sx, sy, sz);
     const T kzy = iso4<T, 1>(F.cnz, c2, sx, sy, sz);
     const T cx = F.cnx[c2], cy = F.cny[c2], cz = F.cnz[c2];
-    F.curv[i1(d, i, j, k)] = (cx * cx - T(1.0)) * kxx + (cy * cy - T(1.0)) * kyy + (cz * cz - T(1.0)) * kzz +
-                             cx * cy * (kxy + kyx) + cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+    return (cx * cx - T(1.0)) * kxx + (cy * cy - T(1.0)) * kyy + (cz * cz - T(1.0)) * kzz + cx * cy * (kxy + kyx) +
+           cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_curvature(Fields<T> F, StepConst<T> P, const uint32_t* __restrict__ list, int n,
+                                                   T* __restrict__ aux) {
+    const int t = xcd_block() * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c2 = i2(d, i, j, k);
+    const T curv = curvature_at(F, c2);
+    F.curv[i1(d, i, j, k)] = curv;
+    aux[4 * (int64_t)t + 0] = F.cnx[c2];
+    aux[4 * (int64_t)t + 1] = F.cny[c2];
+    aux[4 * (int64_t)t + 2] = F.cnz[c2];
+    aux[4 * (int64_t)t + 3] = T(0.5) * P.gamma * curv * F.cnorm[c2];
+}
+
+// Rebuilds the force records from the stored normals and curvature (after an upload).
+template <typename T>
+__global__ __launch_bounds__(256) void k_aux_refresh(Fields<T> F, StepConst<T> P, const uint32_t* __restrict__ list,
+                                                     int n, T* __restrict__ aux) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    unpack_ijk(list[t], i, j, k);
+    const Dims& d = F.d;
+    const int c2 = i2(d, i, j, k);
+    aux[4 * (int64_t)t + 0] = F.cnx[c2];
+    aux[4 * (int64_t)t + 1] = F.cny[c2];
+    aux[4 * (int64_t)t + 2] = F.cnz[c2];
+    aux[4 * (int64_t)t + 3] = T(0.5) * P.gamma * F.curv[i1(d, i, j, k)] * F.cnorm[c2];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -633,22 +572,23 @@ __global__ __launch_bounds__(256) void k_init_eq(Fields<T> F, const double* rho,
     }
 }
 
+
 // ---------------------------------------------------------------------------------------
-// compute_macro_vars (src/Misc.cpp:222-274) and the monitor sums (src/Monitor.cpp:17-80)
+// compute_macro_vars (src/Misc.cpp:222-274) and the monitor sums (src/Monitor.cpp:17-80),
+// over the fluid list (solid nodes contribute exactly zero in the reference).
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void macro_node(const Fields<T>& F, const StepConst<T>& P, int i, int j, int k, double& rho,
-                                           double& u, double& v, double& w) {
-    const Dims& d = F.d;
-    const int c = i1(d, i, j, k), c2 = i2(d, i, j, k);
+__device__ __forceinline__ void macro_node(const Sparse<T>& S, const StepConst<T>& P, int t, double& rho, double& u,
+                                           double& v, double& w) {
     T ft[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) ft[q] = F.pdf[c + d.S1 * q] + F.pdf[c + d.S1 * (q + 19)];
+    for (int q = 0; q < 19; ++q) ft[q] = S.f[q * S.ns + t] + S.f[(q + 19) * S.ns + t];
     T r = ft[0];
 #pragma unroll
     for (int q = 1; q < 19; ++q) r = r + ft[q];
-    const T tmp = T(0.5) * P.gamma * F.curv[c] * F.cnorm[c2];
-    const T fx = tmp * F.cnx[c2], fy = tmp * F.cny[c2], fz = tmp * F.cnz[c2] + P.force_z;
+    const T* a = S.aux + 4 * (int64_t)t;
+    const T tmp = a[3];
+    const T fx = tmp * a[0], fy = tmp * a[1], fz = tmp * a[2] + P.force_z;
     rho = (double)r;
     u = (double)(ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] - T(0.5) * fx);
     v = (double)(ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] - T(0.5) * fy);
@@ -656,38 +596,41 @@ __device__ __forceinline__ void macro_node(const Fields<T>& F, const StepConst<T
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_macro(Fields<T> F, StepConst<T> P, double* rho, double* u, double* v, double* w) {
-    const Dims& d = F.d;
+__global__ __launch_bounds__(256) void k_macro(Sparse<T> S, StepConst<T> P, Dims d, double* rho, double* u, double* v,
+                                               double* w) {
     const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= d.nx * d.ny * d.nz) return;
-    const int i = t % d.nx + 1, j = (t / d.nx) % d.ny + 1, k = t / (d.nx * d.ny) + 1;
+    if (t >= S.nf) return;
+    int i, j, k;
+    unpack_ijk(S.list[t], i, j, k);
     const int c = i1(d, i, j, k);
-    double r = 0, uu = 0, vv = 0, ww = 0;
-    if (F.walls[i2(d, i, j, k)] == 0) macro_node(F, P, i, j, k, r, uu, vv, ww);
+    double r, uu, vv, ww;
+    macro_node(S, P, t, r, uu, vv, ww);
     if (rho) rho[c] = r;
     if (u) u[c] = uu;
     if (v) v[c] = vv;
     if (w) w[c] = ww;
 }
 
-// Partial monitor sums: grid (nsplit, nz); block (b, k) reduces rows [b*rows, (b+1)*rows)
-// of slice k into 10 FP64 values: fl1 fl2 vol1 vol2 mass1 mass2 pre umax2 usq1 usq2.
+// Partial monitor sums: grid (nsplit, nz); block (b, k) reduces its share of the fluid
+// nodes of slice k (list entries zstart[k-1] .. zstart[k]) into 10 FP64 values:
+// fl1 fl2 vol1 vol2 mass1 mass2 pre umax2 usq1 usq2.
 constexpr int MON_VALUES = 10;
 template <typename T>
-__global__ __launch_bounds__(256) void k_monitor_partial(Fields<T> F, StepConst<T> P, int rows, double* __restrict__ part) {
-    const Dims& d = F.d;
-    const int k = blockIdx.y + 1;
-    const int j0 = blockIdx.x * rows + 1, j1 = min(j0 + rows - 1, d.ny);
+__global__ __launch_bounds__(256) void k_monitor_partial(Sparse<T> S, StepConst<T> P, const T* __restrict__ phi, Dims d,
+                                                         const int32_t* __restrict__ zstart, double* __restrict__ part) {
+    const int k = blockIdx.y;
+    const int beg = zstart[k], end = zstart[k + 1];
+    const int chunk = (end - beg + gridDim.x - 1) / gridDim.x;
+    const int b0 = beg + blockIdx.x * chunk, b1 = min(end, b0 + chunk);
     double acc[MON_VALUES];
 #pragma unroll
     for (int m = 0; m < MON_VALUES; ++m) acc[m] = 0.0;
-    const int cnt = (j1 - j0 + 1) * d.nx;
-    for (int t = threadIdx.x; t < cnt; t += 256) {
-        const int i = t % d.nx + 1, j = j0 + t / d.nx;
-        if (F.walls[i2(d, i, j, k)] != 0) continue;
+    for (int t = b0 + threadIdx.x; t < b1; t += 256) {
         double r, u, v, w;
-        macro_node(F, P, i, j, k, r, u, v, w);
-        const double ph = (double)F.phi[i4(d, i, j, k)];
+        macro_node(S, P, t, r, u, v, w);
+        int i, j, kk;
+        unpack_ijk(S.list[t], i, j, kk);
+        const double ph = (double)phi[i4(d, i, j, kk)];
         // per-node terms in the reference's operand order (src/Monitor.cpp:60-67)
         acc[0] += w * 0.5 * (1.0 + ph);
         acc[1] += w * 0.5 * (1.0 - ph);
@@ -714,7 +657,8 @@ __global__ __launch_bounds__(256) void k_monitor_partial(Fields<T> F, StepConst<
         }
         __syncthreads();
     }
-    if (threadIdx.x < MON_VALUES) part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MON_VALUES + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < MON_VALUES)
+        part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MON_VALUES + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Combine the partials of each slice in block order: out[m * nz + k].

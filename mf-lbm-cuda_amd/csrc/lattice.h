@@ -83,6 +83,17 @@ struct NodeLists {
     int n_solid_bnd;
 };
 
+// Fluid-compact population storage (see kernels.inl for the cell layout).
+template <typename T>
+struct Sparse {
+    T* f;                           // 38 * ns, slot (s + 19 g) at offset (s + 19 g) * ns
+    int64_t ns;                     // cells per slot array
+    int nf;                         // interior fluid nodes
+    const int32_t* __restrict__ nb; // 18 * nf
+    const uint32_t* __restrict__ list;  // nf packed coordinates
+    T* aux;                         // 4 * nf force records (cn_x, cn_y, cn_z, 0.5 gamma curv c_norm)
+};
+
 struct BoundaryFlags {
     int kper, jper;
     int open_z;      // kper == 0 and no z walls

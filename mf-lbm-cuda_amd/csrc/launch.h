@@ -8,20 +8,25 @@ namespace mflbm {
 
 template <typename T>
 struct Launch {
-    static void collide(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, bool odd, hipStream_t s);
-    static void periodic(const Fields<T>& F, int kper, int jper, bool odd, hipStream_t s);
-    static void zou_he(const Fields<T>& F, const StepConst<T>& P, int do_in, int do_out, bool after_odd, hipStream_t s);
+    static void collide(const Sparse<T>& S, T* phi, const Dims& d, const StepConst<T>& P, bool odd, hipStream_t s);
+    static void periodic_pdf(const Sparse<T>& S, const int32_t* pairs, int npairs, hipStream_t s);
+    static void periodic_phi(T* phi, const Dims& d, int kper, int jper, hipStream_t s);
+    static void zou_he(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int do_in,
+                       int do_out, bool after_odd, hipStream_t s);
     static void phi_extrap(const Fields<T>& F, const NodeLists& L, hipStream_t s);
     static void normals(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, hipStream_t s);
     static void normal_extrap(const Fields<T>& F, const NodeLists& L, hipStream_t s);
-    static void curvature(const Fields<T>& F, const NodeLists& L, hipStream_t s);
+    static void curvature(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s);
+    static void aux_refresh(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s);
+    static void cells(const Sparse<T>& S, T* dense, const Dims& d, const int32_t* ext_node, const int32_t* ext_off,
+                      int n_ext, bool to_dense, hipStream_t s);
     static void init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, hipStream_t s);
     static void init_eq(const Fields<T>& F, const double* rho, const double* u, const double* v, const double* w,
                         hipStream_t s);
-    static void macro(const Fields<T>& F, const StepConst<T>& P, double* rho, double* u, double* v, double* w,
-                      hipStream_t s);
-    static void monitor(const Fields<T>& F, const StepConst<T>& P, int rows, int nsplit, double* part, double* out,
-                        hipStream_t s);
+    static void macro(const Sparse<T>& S, const StepConst<T>& P, const Dims& d, double* rho, double* u, double* v,
+                      double* w, hipStream_t s);
+    static void monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi, const Dims& d, const int32_t* zstart,
+                        int nsplit, double* part, double* out, hipStream_t s);
     static void to_real(const double* src, T* dst, int64_t n, hipStream_t s);
 };
 

@@ -16,26 +16,29 @@ __global__ __launch_bounds__(256) void k_to_real(const double* __restrict__ src,
 }  // namespace
 
 template <typename T>
-void Launch<T>::collide(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, bool odd, hipStream_t s) {
-    if (L.n_fluid <= 0) return;
-    if (odd) dev::k_collide<T, true><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid);
-    else dev::k_collide<T, false><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid);
+void Launch<T>::collide(const Sparse<T>& S, T* phi, const Dims& d, const StepConst<T>& P, bool odd, hipStream_t s) {
+    if (S.nf <= 0) return;
+    if (odd) dev::k_collide<T, true><<<blocks(S.nf), 256, 0, s>>>(S, phi, d, P);
+    else dev::k_collide<T, false><<<blocks(S.nf), 256, 0, s>>>(S, phi, d, P);
 }
 
 template <typename T>
-void Launch<T>::periodic(const Fields<T>& F, int kper, int jper, bool odd, hipStream_t s) {
-    const int64_t n = (kper ? (int64_t)F.d.nx * F.d.ny : 0) + (jper ? (int64_t)F.d.nx * F.d.nz : 0) +
-                      ((kper && jper) ? F.d.nx : 0);
-    if (n == 0) return;
-    if (odd) dev::k_periodic<T, true><<<blocks(n), 256, 0, s>>>(F, kper, jper);
-    else dev::k_periodic<T, false><<<blocks(n), 256, 0, s>>>(F, kper, jper);
+void Launch<T>::periodic_pdf(const Sparse<T>& S, const int32_t* pairs, int npairs, hipStream_t s) {
+    if (npairs > 0) dev::k_periodic_pdf<T><<<blocks(npairs), 256, 0, s>>>(S, pairs, npairs);
 }
 
 template <typename T>
-void Launch<T>::zou_he(const Fields<T>& F, const StepConst<T>& P, int do_in, int do_out, bool after_odd, hipStream_t s) {
+void Launch<T>::periodic_phi(T* phi, const Dims& d, int kper, int jper, hipStream_t s) {
+    const int64_t n = (kper ? (int64_t)d.nx * d.ny : 0) + (jper ? (int64_t)d.nx * d.nz : 0) + ((kper && jper) ? d.nx : 0);
+    if (n > 0) dev::k_periodic_phi<T><<<blocks(n), 256, 0, s>>>(phi, d, kper, jper);
+}
+
+template <typename T>
+void Launch<T>::zou_he(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int do_in,
+                       int do_out, bool after_odd, hipStream_t s) {
     const int64_t n = (int64_t)F.d.nx * F.d.ny;
-    if (after_odd) dev::k_zou_he<T, true><<<blocks(n), 256, 0, s>>>(F, P, do_in, do_out);
-    else dev::k_zou_he<T, false><<<blocks(n), 256, 0, s>>>(F, P, do_in, do_out);
+    if (after_odd) dev::k_zou_he<T, true><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, do_in, do_out);
+    else dev::k_zou_he<T, false><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, do_in, do_out);
 }
 
 template <typename T>
@@ -54,8 +57,25 @@ void Launch<T>::normal_extrap(const Fields<T>& F, const NodeLists& L, hipStream_
 }
 
 template <typename T>
-void Launch<T>::curvature(const Fields<T>& F, const NodeLists& L, hipStream_t s) {
-    if (L.n_fluid > 0) dev::k_curvature<T><<<blocks(L.n_fluid), 256, 0, s>>>(F, L.fluid, L.n_fluid);
+void Launch<T>::curvature(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s) {
+    if (L.n_fluid > 0) dev::k_curvature<T><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid, aux);
+}
+
+template <typename T>
+void Launch<T>::aux_refresh(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s) {
+    if (L.n_fluid > 0) dev::k_aux_refresh<T><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid, aux);
+}
+
+template <typename T>
+void Launch<T>::cells(const Sparse<T>& S, T* dense, const Dims& d, const int32_t* ext_node, const int32_t* ext_off,
+                      int n_ext, bool to_dense, hipStream_t s) {
+    if (to_dense) {
+        if (S.nf > 0) dev::k_fluid_cells<T, true><<<blocks(S.nf), 256, 0, s>>>(S, dense, d);
+        if (n_ext > 0) dev::k_extra_cells<T, true><<<blocks(n_ext), 256, 0, s>>>(S, dense, d, ext_node, ext_off);
+    } else {
+        if (S.nf > 0) dev::k_fluid_cells<T, false><<<blocks(S.nf), 256, 0, s>>>(S, dense, d);
+        if (n_ext > 0) dev::k_extra_cells<T, false><<<blocks(n_ext), 256, 0, s>>>(S, dense, d, ext_node, ext_off);
+    }
 }
 
 template <typename T>
@@ -70,17 +90,17 @@ void Launch<T>::init_eq(const Fields<T>& F, const double* rho, const double* u, 
 }
 
 template <typename T>
-void Launch<T>::macro(const Fields<T>& F, const StepConst<T>& P, double* rho, double* u, double* v, double* w,
-                      hipStream_t s) {
-    dev::k_macro<T><<<blocks((int64_t)F.d.nx * F.d.ny * F.d.nz), 256, 0, s>>>(F, P, rho, u, v, w);
+void Launch<T>::macro(const Sparse<T>& S, const StepConst<T>& P, const Dims& d, double* rho, double* u, double* v,
+                      double* w, hipStream_t s) {
+    if (S.nf > 0) dev::k_macro<T><<<blocks(S.nf), 256, 0, s>>>(S, P, d, rho, u, v, w);
 }
 
 template <typename T>
-void Launch<T>::monitor(const Fields<T>& F, const StepConst<T>& P, int rows, int nsplit, double* part, double* out,
-                        hipStream_t s) {
-    dim3 grid(nsplit, F.d.nz);
-    dev::k_monitor_partial<T><<<grid, 256, 0, s>>>(F, P, rows, part);
-    dev::k_monitor_combine<T><<<blocks(F.d.nz, 64), 64, 0, s>>>(part, nsplit, F.d.nz, out);
+void Launch<T>::monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi, const Dims& d, const int32_t* zstart,
+                        int nsplit, double* part, double* out, hipStream_t s) {
+    dim3 grid(nsplit, d.nz);
+    dev::k_monitor_partial<T><<<grid, 256, 0, s>>>(S, P, phi, d, zstart, part);
+    dev::k_monitor_combine<T><<<blocks(d.nz, 64), 64, 0, s>>>(part, nsplit, d.nz, out);
 }
 
 template <typename T>

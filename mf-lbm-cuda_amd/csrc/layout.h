@@ -1,0 +1,25 @@
+// Host construction of the fluid-compact population layout (see kernels.inl, Sparse<T>).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "lattice.h"
+
+namespace mflbm {
+
+struct LayoutHost {
+    int nf = 0;                     // interior fluid nodes
+    int64_t ns = 0;                 // cells per slot array (nf + max extra cells of a slot)
+    std::vector<int32_t> nb;        // 18 * nf
+    std::vector<int32_t> ext_node;  // s1 node of every extra cell, grouped by slot
+    std::vector<int32_t> ext_off;   // 20 offsets into ext_node
+    std::vector<int32_t> pairs[2];  // periodic copies (slot, dst, src) for [0] even, [1] odd steps
+    std::vector<int32_t> zmap;      // 6 planes * 19 slots * X1 * Y1 (open z only)
+    std::vector<int32_t> zstart;    // nz + 1 offsets of the z-slices in the fluid list
+};
+
+// fluid: packed coordinates of the interior fluid nodes in (k, j, i) order.
+bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, int jper, int open_z, int zou_in,
+                  int zou_out, LayoutHost& out, std::string& err);
+
+}  // namespace mflbm

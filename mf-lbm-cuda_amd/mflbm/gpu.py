@@ -25,7 +25,7 @@ HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv"
 # Every symbol declared in include/mflbm.h.
 EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_equilibrium",
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
-           "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size"]
+           "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe"]
 
 
 class Params(C.Structure):
@@ -90,6 +90,8 @@ def load():
     L.mflbm_count.restype = C.c_int64
     L.mflbm_field_size.argtypes = [vp, C.c_int32]
     L.mflbm_field_size.restype = C.c_int64
+    L.mflbm_bandwidth_probe.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    L.mflbm_bandwidth_probe.restype = C.c_int
     for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_color_gradient", "mflbm_step", "mflbm_run",
                  "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro", "mflbm_monitor", "mflbm_sync"):
         getattr(L, name).restype = C.c_int
@@ -215,6 +217,12 @@ class Solver:
         self._check(load().mflbm_monitor(self.h, C.byref(m)))
         bufs.update(umax2=m.umax2, usq1=m.usq1, usq2=m.usq2, n_fluid=m.n_fluid)
         return bufs
+
+    def bandwidth_probe(self, nbytes, repeats=10):
+        """Mean device time (ms) of one plain copy of nbytes (read nbytes + write nbytes)."""
+        ms = C.c_double()
+        self._check(load().mflbm_bandwidth_probe(self.h, int(nbytes), int(repeats), C.byref(ms)))
+        return ms.value
 
     def counts(self):
         L = load()
