@@ -14,10 +14,16 @@ solids included, per wall-clock second.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import platform
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -30,8 +36,11 @@ from mflbm import cases, config, gpu  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
+T_START = time.perf_counter()
+
+
 def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s]", *a, file=sys.stderr, flush=True)
 
 
 def cpu_model():
@@ -64,17 +73,117 @@ def cpu_baseline(geo, params, steps):
                       f"OpenMP FP64 oracle, {threads} threads on {cpu_model()}"}
 
 
+PROBE_KERNEL = "k_probe_copy"
+PMC_STEPS = 4  # steps 1..4: two odd and two even collisions
+
+
+def kernel_group(name):
+    """Short name of a step kernel from its demangled name (None for other kernels)."""
+    if PROBE_KERNEL in name:
+        return "probe"
+    for key, short in (("k_collide<double, true>", "collide_odd"), ("k_collide<float, true>", "collide_odd"),
+                       ("k_collide<double, false>", "collide_even"), ("k_collide<float, false>", "collide_even"),
+                       ("k_gradient<", "gradient")):
+        if key in name:
+            return short
+    return None
+
+
+def pmc_child(args):
+    """Run under `rocprofv3 --pmc`: a few plain steps of the workload and one probe copy."""
+    prec = gpu.FP64 if args.precision == "fp64" else gpu.FP32
+    geo = np.load(args.pmc_child, allow_pickle=False)
+    n = geo.shape[0]
+    _, ctrl = cases.config2(n=(n, n, n), geometry=False)
+    sim = gpu.Solver(config.derive(ctrl), geo, prec)
+    sim.setup()
+    for t in range(1, PMC_STEPS + 1):
+        sim.step(t)
+    sim.bandwidth_probe(args.probe_mib << 20, repeats=1)
+    sim.sync()
+    sim.close()
+    print("pmc child done", flush=True)
+
+
+def counter_pass(counter, geo_path, args, timeout=240):
+    """Counter values (KiB per dispatch) of one rocprofv3 --pmc pass, grouped by kernel."""
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = tempfile.mkdtemp(prefix="mflbm_pmc_", dir="/tmp")
+    cmd = [rp, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+           sys.executable, os.path.abspath(__file__), "--pmc-child", geo_path, "--precision", args.precision,
+           "--probe-mib", str(args.probe_mib)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    log(f"rocprofv3 --pmc {counter} pass")
+    proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                            start_new_session=True, text=True)
+    try:
+        text, _ = proc.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, signal.SIGKILL)
+        proc.communicate()
+        shutil.rmtree(out, ignore_errors=True)
+        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    # rocprofv3 itself may end with SIGSEGV after a complete run; the child's last line and
+    # the result file decide
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if "pmc child done" not in text or not files:
+        shutil.rmtree(out, ignore_errors=True)
+        raise RuntimeError(f"rocprofv3 --pmc {counter} failed (rc {proc.returncode}): {text[-400:]}")
+    vals = {}
+    with open(files[0], newline="") as fh:
+        for row in csv.DictReader(fh):
+            if row["Counter_Name"] != counter:
+                continue
+            g = kernel_group(row["Kernel_Name"])
+            if g:
+                vals.setdefault(g, []).append(float(row["Counter_Value"]))
+    shutil.rmtree(out, ignore_errors=True)
+    return vals
+
+
+def measure_traffic(geo, args):
+    """HBM bytes per dispatch of the step kernels from FETCH_SIZE and WRITE_SIZE passes,
+    each scaled so that the plain copy of known size reads and writes exactly its bytes."""
+    tmp = tempfile.mkdtemp(prefix="mflbm_geo_", dir="/tmp")
+    geo_path = os.path.join(tmp, "geo.npy")
+    np.save(geo_path, geo)
+    try:
+        f = counter_pass("FETCH_SIZE", geo_path, args)
+        w = counter_pass("WRITE_SIZE", geo_path, args)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    nbytes = args.probe_mib << 20
+    sf = nbytes / (f["probe"][-1] * 1024.0)
+    sw = nbytes / (w["probe"][-1] * 1024.0)
+    res = {"fetch_scale": round(sf, 4), "write_scale": round(sw, 4)}
+    for g in ("collide_odd", "collide_even", "gradient"):
+        if g in f and g in w:
+            fb = float(np.median(f[g])) * 1024.0 * sf
+            wb = float(np.median(w[g])) * 1024.0 * sw
+            res[g] = {"fetch": round(fb), "write": round(wb), "total": round(fb + wb)}
+    log("traffic per dispatch (GB): " + ", ".join(f"{g} {res[g]['total'] / 1e9:.3f}" for g in
+                                                  ("collide_odd", "collide_even", "gradient") if g in res)
+        + f"; counter scales fetch x{sf:.3f} write x{sw:.3f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--n", type=int, default=256, help="cubic lattice edge")
     ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--probe-mib", type=int, default=2048, help="plain-copy calibration size (0 = skip)")
+    ap.add_argument("--probe-mib", type=int, default=1024, help="plain-copy calibration size (0 = skip)")
+    ap.add_argument("--traffic", action=argparse.BooleanOptionalAction, default=True,
+                    help="measure HBM bytes per kernel with two rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU z-slab "
               "decomposition is a design note, not built.", file=sys.stderr)
@@ -86,6 +195,14 @@ def main():
     log(f"generating {n}^3 sphere pack")
     geo, ctrl = cases.config2(n=(n, n, n))
     params = config.derive(ctrl)
+
+    traffic = None
+    if args.traffic and args.probe_mib > 0:
+        try:  # before this process touches the GPU
+            traffic = measure_traffic(geo, args)
+        except (RuntimeError, OSError, KeyError, IndexError) as e:
+            log(f"traffic measurement skipped: {e}")
+
     log("creating device state")
     sim = gpu.Solver(params, geo, prec)
     sim.setup()
@@ -105,17 +222,23 @@ def main():
     mflups = n_fluid * args.steps / dt / 1e6
     log(f"{args.steps} steps in {dt:.3f} s: {mlups:.1f} MLUPS, {mflups:.1f} fluid-MLUPS")
 
-    # per-kernel device times (events on the solver's stream)
+    # per-kernel device times: HIP events on the solver's stream, around each kernel group
     kt = sim.profile(ntime0 + args.steps, args.profile_steps)
-    coll_ms = (kt["collide_odd_ms"] + kt["collide_even_ms"]) / max(1, kt["n_odd"] + kt["n_even"])
-    step_ms = sum(kt[k] for k in ("collide_odd_ms", "collide_even_ms", "boundary_ms", "phi_extrap_ms", "normals_ms",
-                                  "normal_extrap_ms", "curvature_ms")) / max(1, kt["n_odd"] + kt["n_even"])
+    n_steps = max(1, kt["n_odd"] + kt["n_even"])
+    odd_ms = kt["collide_odd_ms"] / max(1, kt["n_odd"])
+    even_ms = kt["collide_even_ms"] / max(1, kt["n_even"])
+    coll_ms = 0.5 * (odd_ms + even_ms)
+    grad_ms = kt["gradient_ms"] / n_steps
+    bnd_ms = kt["boundary_ms"] / n_steps
+    step_ms = coll_ms + bnd_ms + grad_ms
     # algorithmic bytes of one collide launch: 38 slots read + 38 written + phi written per
-    # fluid node, plus one mask byte per lattice node
-    coll_bytes = n_fluid * (76 * s + s) + n_all * 1
+    # fluid node, plus the 4-value force record read per fluid node
+    coll_bytes = n_fluid * (76 * s + s + 4 * s)
     achieved = coll_bytes / (coll_ms * 1e-3) / 1e9
-    step_bytes = n_fluid * 78 * s + n_all
-    log(f"collide {coll_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), step (kernels) {step_ms:.3f} ms")
+    # step: collide + gradient chain (phi read and force records written per fluid node)
+    step_bytes = coll_bytes + n_fluid * (s + 4 * s)
+    log(f"collide odd {odd_ms:.3f} ms, even {even_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), "
+        f"boundary {bnd_ms:.3f} ms, gradient {grad_ms:.3f} ms, step (kernels) {step_ms:.3f} ms")
 
     probe = None
     if args.probe_mib > 0:
@@ -123,6 +246,7 @@ def main():
         pms = sim.bandwidth_probe(nbytes, repeats=10)
         probe = {"bytes": nbytes, "ms": round(pms, 4), "gbs": round(2 * nbytes / (pms * 1e-3) / 1e9, 1)}
         log(f"plain copy of {args.probe_mib} MiB: {pms:.3f} ms = {probe['gbs']} GB/s (read+write)")
+    sim.close()
 
     cpu = None
     if args.cpu_steps > 0:
@@ -130,6 +254,9 @@ def main():
         cpu = cpu_baseline(geo, params, args.cpu_steps)
         log(f"CPU baseline {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
 
+    coll_traffic = None
+    if traffic and "collide_odd" in traffic and "collide_even" in traffic:
+        coll_traffic = round(0.5 * (traffic["collide_odd"]["total"] + traffic["collide_even"]["total"]))
     out = {
         "metric": "MLUPS", "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
@@ -139,11 +266,14 @@ def main():
                                f"{'FP64' if prec == gpu.FP64 else 'FP32'}",
                    "lattice": [n, n, n], "porosity": round(1 - float(geo.mean()), 4), "fluid_nodes": int(n_fluid)},
         "fluid_mlups": round(mflups, 3),
-        "step_roofline_frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "kernel_ms": {k: round(v / max(1, kt["n_odd"] + kt["n_even"]), 4) for k, v in kt.items() if k.endswith("_ms")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_collide (odd+even average)"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": coll_traffic,
+                     "kernel": "k_collide (mean of the odd and even launches)",
+                     "algorithmic_bytes": int(coll_bytes)},
+        "step_roofline_frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {"collide_odd": round(odd_ms, 4), "collide_even": round(even_ms, 4),
+                      "boundary": round(bnd_ms, 4), "gradient": round(grad_ms, 4), "step": round(step_ms, 4)},
+        "traffic_bytes": traffic,
         "copy_probe": probe,
         "cpu_baseline": cpu,
     }

@@ -41,8 +41,15 @@ typedef struct mflbm_params {
     double lbm_gamma, force_z, la_nui1, la_nui2, lbm_beta, phi_inlet, sa_inject, cos_theta;
     double rho_in, rho_out, relaxation;
     int32_t precision;           /* enum mflbm_precision */
-    int32_t reserved;
+    int32_t flags;               /* MFLBM_FLAG_* */
 } mflbm_params;
+
+/* Run the colour-gradient chain as four kernels per step, one per reference kernel
+ * (phi extrapolation, normals + wetting, normal extrapolation, curvature), writing the
+ * dense fields every step.  Default (0): one fused kernel that writes only what the next
+ * collision reads; the dense fields are brought up to date when downloaded.  Both give
+ * identical results. */
+#define MFLBM_FLAG_SPLIT_GRADIENT 1
 
 /* Fields for upload/download. */
 enum mflbm_field {
@@ -64,9 +71,11 @@ typedef struct mflbm_monitor_out {
     int64_t n_fluid;              /* fluid nodes in the interior */
 } mflbm_monitor_out;
 
-/* Per-kernel device time of a profiled run (milliseconds, summed over steps). */
+/* Device time of the kernel groups of a profiled run (milliseconds, summed over steps;
+ * divide collide_odd_ms by n_odd, collide_even_ms by n_even, the others by n_odd + n_even).
+ * boundary = periodic copies + Zou-He; gradient = the colour-gradient chain. */
 typedef struct mflbm_kernel_times {
-    double collide_odd_ms, collide_even_ms, boundary_ms, phi_extrap_ms, normals_ms, normal_extrap_ms, curvature_ms;
+    double collide_odd_ms, collide_even_ms, boundary_ms, gradient_ms;
     int32_t n_odd, n_even;
 } mflbm_kernel_times;
 

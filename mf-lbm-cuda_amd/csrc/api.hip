@@ -2,14 +2,18 @@
 // step sequence (captured into HIP graphs), monitor reductions and field transfers.
 //
 // Step sequence per time step (src/main_iteration_GPU.cu:1890-2055):
-//   collide (odd/even)  ->  periodic copies  ->  Zou-He in/outlet  ->
-//   phi extrapolation  ->  normals + wetting  ->  normal extrapolation  ->  curvature
+//   collide (odd/even)  ->  periodic copies  ->  Zou-He in/outlet  ->  gradient chain
+// The gradient chain (phi extrapolation, normals + wetting, normal extrapolation,
+// curvature) runs as one fused tiled kernel that writes only the force records the next
+// collision reads; the split kernels (one per reference kernel) fill the dense gradient
+// fields when they are downloaded, or run every step with MFLBM_FLAG_SPLIT_GRADIENT.
 // The reference synchronises the device after every launch; here all launches of a step
 // are stream-ordered and replayed from a captured graph.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,6 +51,11 @@ struct mflbm_handle {
     int32_t *ext_node = nullptr, *ext_off = nullptr;
     int n_ext = 0;
     int32_t* zstart = nullptr;
+    // fused gradient chain
+    int32_t* tag = nullptr;     // S4 node tags (lattice.h GradArgs)
+    int grad_shape = 0, tiles_x = 0, tiles_y = 0, kchunk = 0;
+    bool split_gradient = false;
+    bool grad_dirty = false;    // dense phi / cn / c_norm / curv behind the step state
     // monitor scratch
     double *mon_part = nullptr, *mon_out = nullptr;
     int mon_nsplit = 0;
@@ -56,6 +65,8 @@ struct mflbm_handle {
 };
 
 namespace {
+
+constexpr int64_t SLOT_SKEW_DEFAULT = 0;
 
 size_t esize(const mflbm_handle* h) { return h->prec == MFLBM_FP64 ? sizeof(double) : sizeof(float); }
 
@@ -108,6 +119,18 @@ Sparse<T> sparse(const mflbm_handle* h) {
     return S;
 }
 
+template <typename T>
+GradArgs<T> grad_args(const mflbm_handle* h) {
+    GradArgs<T> A;
+    A.d = h->d;
+    A.phi = (T*)h->phi;
+    A.tag = h->tag;
+    A.snx = (const T*)h->snx; A.sny = (const T*)h->sny; A.snz = (const T*)h->snz;
+    A.aux = (T*)h->aux;
+    A.shape = h->grad_shape; A.tiles_x = h->tiles_x; A.tiles_y = h->tiles_y; A.kchunk = h->kchunk;
+    return A;
+}
+
 NodeLists lists(const mflbm_handle* h) {
     NodeLists L;
     L.fluid = h->l_fluid; L.n_fluid = h->n_fluid;
@@ -137,13 +160,14 @@ void enqueue_step(mflbm_handle* h, bool odd, hipEvent_t* ev) {
         if (in || out) Launch<T>::zou_he(S, F, P, h->zmap, in, out, odd, s);
     }
     mark();
-    Launch<T>::phi_extrap(F, L, s);
-    mark();
-    Launch<T>::normals(F, P, L, s);
-    mark();
-    Launch<T>::normal_extrap(F, L, s);
-    mark();
-    Launch<T>::curvature(F, P, L, (T*)h->aux, s);
+    if (h->split_gradient) {
+        Launch<T>::phi_extrap(F, L, s);
+        Launch<T>::normals(F, P, L, s);
+        Launch<T>::normal_extrap(F, L, s);
+        Launch<T>::curvature(F, P, L, (T*)h->aux, s);
+    } else {
+        Launch<T>::gradient(grad_args<T>(h), P, s);
+    }
     mark();
 }
 
@@ -178,12 +202,26 @@ void step_dispatch(mflbm_handle* h, bool odd, hipEvent_t* ev) {
     else enqueue_step<float>(h, odd, ev);
 }
 
+// Brings the dense phi (solid-boundary nodes of the ghost layers), cn, c_norm and curv up
+// to the step state with the split kernels.  The fused step leaves them behind; they are
+// functions of phi alone, and the split kernels also rewrite the force records with the
+// same values.
+int materialize_gradient(mflbm_handle* h) {
+    if (!h->grad_dirty) return 0;
+    if (h->prec == MFLBM_FP64) enqueue_gradient<double>(h);
+    else enqueue_gradient<float>(h);
+    h->grad_dirty = false;
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
 void free_all(mflbm_handle* h) {
     for (int g = 0; g < 2; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out,
-                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->ext_node, h->ext_off, h->zstart};
+                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->ext_node, h->ext_off, h->zstart,
+                    h->tag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
@@ -191,6 +229,7 @@ void free_all(mflbm_handle* h) {
     h->mon_part = h->mon_out = nullptr;
     h->cells = h->aux = nullptr;
     h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->ext_node = h->ext_off = h->zstart = nullptr;
+    h->tag = nullptr;
     if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
 }
 
@@ -347,13 +386,53 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     HIPCHK(h, upload_list(&h->l_fluid, lf));
     HIPCHK(h, upload_list(&h->l_fluid_ext, lfe));
     HIPCHK(h, upload_list(&h->l_sb, lsb));
+    auto upload_i32_raw = [&](int32_t** dst, const std::vector<int32_t>& v) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(int32_t));
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    };
+
+    // fused gradient chain: node tags with the force-record index of interior fluid nodes
+    {
+        if ((int64_t)lf.size() >= (int64_t(1) << 29)) return fail(h, "too many fluid nodes for the node tags");
+        std::vector<int32_t> tag(d.S4);
+        for (int64_t c = 0; c < d.S4; ++c) tag[c] = (int32_t)(wtype[c] & 3);
+        for (size_t t = 0; t < lf.size(); ++t) {
+            int i, j, k;
+            unpack_ijk(lf[t], i, j, k);
+            tag[i4(d, i, j, k)] |= (int32_t)(t << 2);
+        }
+        HIPCHK(h, upload_i32_raw(&h->tag, tag));
+        h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
+        h->grad_shape = GRAD_SHAPE_DEFAULT;
+        if (const char* env = std::getenv("MFLBM_GRAD_SHAPE")) h->grad_shape = std::atoi(env);
+        if (h->grad_shape < 0 || h->grad_shape >= GRAD_NSHAPES) return fail(h, "MFLBM_GRAD_SHAPE out of range");
+        const GradShape gs = GRAD_SHAPES[h->grad_shape];
+        h->tiles_x = (d.nx + gs.tx - 1) / gs.tx;
+        h->tiles_y = (d.ny + gs.ty - 1) / gs.ty;
+        // z-run per workgroup: long enough to amortise the 8 halo planes, short enough to
+        // give each of the 256 CUs two workgroups (the LDS use allows two);
+        // MFLBM_GRAD_KCHUNK and MFLBM_GRAD_SHAPE override (tuning)
+        const int tiles = h->tiles_x * h->tiles_y;
+        int kc = std::max(16, (d.nz * tiles + 511) / 512);
+        if (const char* env = std::getenv("MFLBM_GRAD_KCHUNK")) kc = std::max(1, std::atoi(env));
+        h->kchunk = std::min(kc, d.nz);
+    }
 
     // fluid-compact population cells
     LayoutHost lay;
     std::string lerr;
     if (!build_layout(d, lf, (int)p.kper, (int)p.jper, h->open_z, p.inlet_bc == 2, p.outlet_bc == 2, lay, lerr))
         return fail(h, "layout: %s", lerr.c_str());
-    h->ns = lay.ns;
+    // slot-array stride: the cells a step uses (fluid nodes + extra cells), rounded up to a
+    // 4 KiB multiple plus a skew, so that the 38 streams of one node do not start at the
+    // same offset within the HBM channel interleave; MFLBM_SLOT_SKEW (elements) overrides
+    {
+        int64_t skew = SLOT_SKEW_DEFAULT;
+        if (const char* env = std::getenv("MFLBM_SLOT_SKEW")) skew = std::max(0, std::atoi(env));
+        const int64_t q = 4096 / (int64_t)es;
+        h->ns = (lay.ns + q - 1) / q * q + skew;
+    }
     h->n_ext = lay.ext_off[19];
     HIPCHK(h, hipMalloc(&h->cells, 38 * (size_t)h->ns * es));
     HIPCHK(h, hipMemsetAsync(h->cells, 0, 38 * (size_t)h->ns * es, h->stream));
@@ -424,6 +503,7 @@ int mflbm_color_gradient(mflbm_handle* h) {
     if (!h) return -1;
     if (h->prec == MFLBM_FP64) enqueue_gradient<double>(h);
     else enqueue_gradient<float>(h);
+    h->grad_dirty = false;
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
@@ -432,6 +512,7 @@ int mflbm_color_gradient(mflbm_handle* h) {
 int mflbm_step(mflbm_handle* h, int32_t ntime) {
     if (!h) return -1;
     step_dispatch(h, (ntime % 2) != 0, nullptr);
+    h->grad_dirty = !h->split_gradient;
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
@@ -442,6 +523,7 @@ int mflbm_run(mflbm_handle* h, int32_t ntime0, int32_t nsteps) {
     if (nsteps <= 0) return 0;
     if (build_graphs(h)) return -1;
     for (int32_t t = ntime0; t < ntime0 + nsteps; ++t) HIPCHK(h, hipGraphLaunch(h->graph[(t % 2) != 0], h->stream));
+    h->grad_dirty = !h->split_gradient;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -449,7 +531,7 @@ int mflbm_run(mflbm_handle* h, int32_t ntime0, int32_t nsteps) {
 int mflbm_profile(mflbm_handle* h, int32_t ntime0, int32_t nsteps, mflbm_kernel_times* out) {
     if (!h || !out) return -1;
     std::memset(out, 0, sizeof(*out));
-    constexpr int NE = 7;
+    constexpr int NE = 4;
     hipEvent_t ev[NE];
     for (int e = 0; e < NE; ++e) HIPCHK(h, hipEventCreate(&ev[e]));
     for (int32_t t = ntime0; t < ntime0 + nsteps; ++t) {
@@ -462,11 +544,9 @@ int mflbm_profile(mflbm_handle* h, int32_t ntime0, int32_t nsteps, mflbm_kernel_
         if (odd) { out->collide_odd_ms += ms[0]; out->n_odd++; }
         else { out->collide_even_ms += ms[0]; out->n_even++; }
         out->boundary_ms += ms[1];
-        out->phi_extrap_ms += ms[2];
-        out->normals_ms += ms[3];
-        out->normal_extrap_ms += ms[4];
-        out->curvature_ms += ms[5];
+        out->gradient_ms += ms[2];
     }
+    h->grad_dirty = !h->split_gradient;
     for (int e = 0; e < NE; ++e) (void)hipEventDestroy(ev[e]);
     HIPCHK(h, hipGetLastError());
     return 0;
@@ -477,6 +557,9 @@ int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes) {
     int64_t n; size_t es; void* ptr;
     if (field_info(h, field, &n, &es, &ptr)) return fail(h, "unknown field %d", field);
     if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
+    if (field == MFLBM_PHI || field == MFLBM_CN_X || field == MFLBM_CN_Y || field == MFLBM_CN_Z ||
+        field == MFLBM_C_NORM || field == MFLBM_CURV)
+        if (materialize_gradient(h)) return -1;
     if (field == MFLBM_PDF) {  // scatter the compact cells over the dense copy first
         cells_transfer(h, true);
         HIPCHK(h, hipGetLastError());
@@ -492,6 +575,7 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
     int64_t n; size_t es; void* ptr;
     if (field_info(h, field, &n, &es, &ptr)) return fail(h, "unknown field %d", field);
     if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
+    if (materialize_gradient(h)) return -1;  // the fields not uploaded keep the step state
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
     if (field == MFLBM_PDF) cells_transfer(h, false);

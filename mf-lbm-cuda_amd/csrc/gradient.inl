@@ -1,0 +1,314 @@
+// Fused colour-gradient chain: phi extrapolation to solid-boundary nodes (K3), interface
+// normals with the wetting correction (K4+K5), normal extrapolation to solid-boundary
+// nodes (K6) and curvature (K7), producing the force records of the next collision in one
+// pass over phi (src/main_iteration_GPU.cu:732-1003 run as four dependent kernels there).
+//
+// One workgroup owns a TX x TY column of interior nodes over a run of `kchunk` z-planes and
+// marches up z.  Each stage needs its input one node further out than its output, so the
+// loaded planes carry a 4-node halo in x and y, and the stages trail the load front:
+//   load    plane p     phi, tag                          (TX+8) x (TY+8)
+//   K3      plane p-1   phi at solid-boundary nodes       (TX+6) x (TY+6)
+//   K4+K5   plane p-2   normals + c_norm at fluid nodes   (TX+4) x (TY+4)
+//   K6      plane p-3   normals at solid-boundary nodes   (TX+2) x (TY+2)
+//   K7      plane p-4   curvature -> force record         TX x TY
+// Ring buffers in LDS keep the planes still in use.  K3 and K6 work in place: they write
+// only solid-boundary entries and read only fluid entries, and the D3Q19 neighbours of a
+// fluid node are fluid or solid-boundary nodes by construction of walls_type.
+//
+// Every value goes through the same device functions as the split kernels (iso4_at,
+// interface_normal, curvature_of, the K3/K6 loops in q order), so both paths round alike.
+// The phi written back is the solid-boundary phi of the owned planes (plus the inlet ghost
+// plane k = 0 for the first z-run): the only solid-boundary values a later kernel of the
+// step reads (Zou-He inlet/outlet and the periodic copies).  The full dense gradient
+// fields are produced on request by the split kernels (they are functions of phi alone).
+#pragma once
+#include "kernels.inl"
+
+namespace mflbm {
+namespace dev {
+
+
+// v if f, else +0.0: a bit mask, so that the compiler cannot turn the weighted sums into
+// branches around their loads (it does for a select).
+__device__ __forceinline__ double keep_if(bool f, double v) {
+    return __longlong_as_double(__double_as_longlong(v) & (f ? ~0ll : 0ll));
+}
+__device__ __forceinline__ float keep_if(bool f, float v) { return __int_as_float(__float_as_int(v) & (f ? ~0 : 0)); }
+
+// Workgroup barrier for data exchanged through LDS only.  __syncthreads() also waits for
+// every outstanding global load of the wave (its release fence covers all memory), which
+// would expose the latency of the next plane's loads at every stage boundary.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename T, int TX, int TY, int GRAD_THREADS>
+__global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepConst<T> P) {
+    constexpr int GRAD_WAVES = GRAD_THREADS / 64;
+    constexpr int H = 4;
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H, NP = WX * WY;  // loaded plane
+    constexpr int VX = TX + 4, VY = TY + 4, NV = VX * VY;          // normals plane (halo 2)
+    constexpr int NO = TX * TY;                                    // owned plane
+    constexpr int NL = (NP + GRAD_THREADS - 1) / GRAD_THREADS;
+    constexpr int SEG = (NP + GRAD_WAVES - 1) / GRAD_WAVES;        // list segment per wave
+    constexpr int NSUB = (SEG + 63) / 64;
+    __shared__ T s_phi[4][NP];
+    __shared__ T s_cn[4][3][NV];
+    __shared__ T s_cnorm[3][NO];
+    __shared__ int8_t s_tag[6][NP];
+    // work lists of the four stages (K3, K4, K6, K7), double-buffered by iteration parity:
+    // loaded-plane entries that a stage computes, one segment per wave
+    __shared__ int16_t s_list[2][4][GRAD_WAVES * SEG];
+    __shared__ int s_cnt[2][4][GRAD_WAVES];
+
+    const Dims& d = A.d;
+    const int b = xcd_block();
+    const int tx = b % A.tiles_x, ty = (b / A.tiles_x) % A.tiles_y, tz = b / (A.tiles_x * A.tiles_y);
+    const int ib = 1 + tx * TX - H, jb = 1 + ty * TY - H;  // node of loaded-plane entry (0, 0)
+    const int k0 = 1 + tz * A.kchunk, k1 = min(d.nz, k0 + A.kchunk - 1);
+    const int kw0 = tz == 0 ? 0 : k0;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nxo = min(TX, d.nx - (ib + H) + 1), nyo = min(TY, d.ny - (jb + H) + 1);  // owned extent
+
+    auto s4 = [](int p) { return (p + 8) & 3; };
+    auto s6 = [](int p) { return (p + 12) % 6; };
+    auto s3 = [](int p) { return (p + 12) % 3; };
+
+    // register staging of the next plane: its loads are in flight while the stages run
+    // (kept raw until the store, so that nothing waits for them early)
+    T pv[NL];
+    int32_t tv[NL];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int e = tid + m * GRAD_THREADS;
+            const int gi = ib + e % WX, gj = jb + e / WX;
+            if (e < NP && gi <= d.nx + 4 && gj <= d.ny + 4) {
+                const int c = i4(d, gi, gj, p);
+                pv[m] = A.phi[c];
+                tv[m] = A.tag[c];
+            } else {
+                pv[m] = T(0);
+                tv[m] = TAG_SOLID;
+            }
+        }
+    };
+    // i-th entry of a stage list (segments concatenated in wave order)
+    // i-th entry of a stage list; clamped into the plane so that no address formed from
+    // it can leave the arrays (callers only ask for i below the list total)
+    auto item = [&](const int16_t* list, const int* cnt, int i) -> int {
+        const int e = list_item<GRAD_WAVES>(list, cnt, SEG, i);
+        return e < 0 ? 0 : (e >= NP ? NP - 1 : e);
+    };
+    auto total = [](const int* cnt) {
+        int t = 0;
+#pragma unroll
+        for (int w = 0; w < GRAD_WAVES; ++w) t += cnt[w];
+        return t;
+    };
+
+    fetch(k0 - H);
+    for (int p = k0 - H; p <= k1 + H; ++p) {
+        const int par = p & 1;
+        {
+            T* ph = s_phi[s4(p)];
+            int8_t* tg = s_tag[s6(p)];
+#pragma unroll
+            for (int m = 0; m < NL; ++m) {
+                const int e = tid + m * GRAD_THREADS;
+                if (e < NP) { ph[e] = pv[m]; tg[e] = (int8_t)(tv[m] & 3); }
+            }
+        }
+        if (p < k1 + H) fetch(p + 1);
+        // stage lists of this iteration, from tags loaded in earlier iterations
+        {
+            const int8_t* t3 = s_tag[s6(p - 1)];
+            const int8_t* t4 = s_tag[s6(p - 2)];
+            const int8_t* t6 = s_tag[s6(p - 3)];
+            const int8_t* t7 = s_tag[s6(p - 4)];
+            int c3 = 0, c4 = 0, c6 = 0, c7 = 0;
+            int16_t* l3 = &s_list[par][0][wave * SEG];
+            int16_t* l4 = &s_list[par][1][wave * SEG];
+            int16_t* l6 = &s_list[par][2][wave * SEG];
+            int16_t* l7 = &s_list[par][3][wave * SEG];
+#pragma unroll
+            for (int sb = 0; sb < NSUB; ++sb) {
+                const int j = sb * 64 + lane;
+                const int e = wave * SEG + j;
+                const bool in = j < SEG && e < NP;
+                const int x = e % WX, y = e / WX;
+                const bool r3 = in && x >= 1 && x < WX - 1 && y >= 1 && y < WY - 1;
+                const bool r2 = in && x >= 2 && x < WX - 2 && y >= 2 && y < WY - 2;
+                const bool r1 = in && x >= 3 && x < WX - 3 && y >= 3 && y < WY - 3;
+                const bool r0 = in && x >= H && x < H + nxo && y >= H && y < H + nyo;
+                const bool f3 = r3 && t3[e] == TAG_SOLID_BND;
+                const bool f4 = r2 && tag_fluid(t4[e]);
+                const bool f6 = r1 && t6[e] == TAG_SOLID_BND;
+                const bool f7 = r0 && tag_fluid(t7[e]);
+                auto push = [&](bool f, int16_t* l, int& c) {
+                    const uint64_t m = __ballot(f);
+                    const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (f) l[c + pos] = (int16_t)e;
+                    c += __popcll(m);
+                };
+                push(f3, l3, c3);
+                push(f4, l4, c4);
+                push(f6, l6, c6);
+                push(f7, l7, c7);
+            }
+            if (lane == 0) {
+                s_cnt[par][0][wave] = c3; s_cnt[par][1][wave] = c4;
+                s_cnt[par][2][wave] = c6; s_cnt[par][3][wave] = c7;
+            }
+        }
+        lds_barrier();
+
+        // Global reads of the K4 and K7 stages, issued now so that their latency hides
+        // behind K3: the solid normals of the fluid-boundary nodes of plane p-2 and the
+        // force-record index of the owned fluid nodes of plane p-4.
+        constexpr int N4 = (NV + GRAD_THREADS - 1) / GRAD_THREADS;
+        T wn[N4][3];
+        int32_t rec = 0;
+        if (p - 2 >= k0 - 2) {
+            const int q = p - 2;
+            const int8_t* tc = s_tag[s6(q)];
+            const int16_t* list = s_list[par][1];
+            const int* cnt = s_cnt[par][1];
+            const int n = total(cnt);
+#pragma unroll
+            for (int m = 0; m < N4; ++m) {
+                const int i = tid + m * GRAD_THREADS;
+                wn[m][0] = wn[m][1] = wn[m][2] = T(0);
+                if (i < n) {
+                    const int e = item(list, cnt, i);
+                    if (tc[e] == TAG_WET) {
+                        const int c = i4(d, ib + e % WX, jb + e / WX, q);
+                        wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
+                    }
+                }
+            }
+        }
+        if (p - 4 >= k0) {
+            const int* cnt = s_cnt[par][3];
+            if (tid < total(cnt)) {
+                const int e = item(s_list[par][3], cnt, tid), x = e % WX, y = e / WX;
+                if (x >= H && x < H + nxo && y >= H && y < H + nyo)  // owned (the list holds only such)
+                    rec = A.tag[i4(d, ib + x, jb + y, p - 4)] >> 2;
+            }
+        }
+
+        // K3: phi at solid-boundary nodes of plane q (src/main_iteration_GPU.cu:732-755)
+        if (p - 1 >= k0 - 3) {
+            const int q = p - 1;
+            T* pc = s_phi[s4(q)];
+            const T* pl[3] = {s_phi[s4(q - 1)], pc, s_phi[s4(q + 1)]};
+            const int8_t* tl[3] = {s_tag[s6(q - 1)], s_tag[s6(q)], s_tag[s6(q + 1)]};
+            const bool wb = q >= kw0 && q <= k1;
+            const int16_t* list = s_list[par][0];
+            const int* cnt = s_cnt[par][0];
+            const int n = total(cnt);
+            for (int i = tid; i < n; i += GRAD_THREADS) {
+                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                // branch-free sums: a non-fluid neighbour adds +0 * w = +0, which leaves
+                // the (never negative-zero) partial sums unchanged
+                T ps = T(0), ws = T(0);
+#pragma unroll
+                for (int qq = 1; qq < 19; ++qq) {
+                    const int o = e + EX[qq] + WX * EY[qq];
+                    const bool f = tag_fluid(tl[1 + EZ[qq]][o]);
+                    ps += keep_if(f, pl[1 + EZ[qq]][o]) * wq<T>(qq);
+                    ws += keep_if(f, wq<T>(qq));
+                }
+                const T v = ps / ws;
+                pc[e] = v;
+                if (wb && x >= H && x < H + nxo && y >= H && y < H + nyo) A.phi[i4(d, ib + x, jb + y, q)] = v;
+            }
+        }
+        lds_barrier();
+
+        // K4 + K5: normals and c_norm at fluid nodes of plane q (src/main_iteration_GPU.cu:757-878)
+        if (p - 2 >= k0 - 2) {
+            const int q = p - 2;
+            const T* pl[3] = {s_phi[s4(q - 1)], s_phi[s4(q)], s_phi[s4(q + 1)]};
+            const int8_t* tc = s_tag[s6(q)];
+            T* cn = &s_cn[s4(q)][0][0];
+            T* cno = s_cnorm[s3(q)];
+            const int16_t* list = s_list[par][1];
+            const int* cnt = s_cnt[par][1];
+            const int n = total(cnt);
+#pragma unroll
+            for (int m = 0; m < N4; ++m) {
+                const int i = tid + m * GRAD_THREADS;
+                if (i >= n) break;
+                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                const int r = (x - 2) + VX * (y - 2);
+                auto a = [&](int dx, int dy, int dz) -> T { return pl[1 + dz][e + dx + WX * dy]; };
+                const T cx = iso4_at<T, 0>(a);
+                const T cy = iso4_at<T, 1>(a);
+                const T cz = iso4_at<T, 2>(a);
+                T vx, vy, vz;
+                const T cnorm = interface_normal<T>(
+                    cx, cy, cz, tc[e] == TAG_WET, P.cos_theta,
+                    [&](T& nwx, T& nwy, T& nwz) { nwx = wn[m][0]; nwy = wn[m][1]; nwz = wn[m][2]; },
+                    vx, vy, vz);
+                cn[r] = vx; cn[NV + r] = vy; cn[2 * NV + r] = vz;
+                if (x >= H && x < H + TX && y >= H && y < H + TY) cno[(x - H) + TX * (y - H)] = cnorm;
+            }
+        }
+        lds_barrier();
+
+        // K6: normals at solid-boundary nodes of plane q (src/main_iteration_GPU.cu:880-906)
+        if (p - 3 >= k0 - 1) {
+            const int q = p - 3;
+            const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
+            const int8_t* tl[3] = {s_tag[s6(q - 1)], s_tag[s6(q)], s_tag[s6(q + 1)]};
+            T* cn = &s_cn[s4(q)][0][0];
+            const int16_t* list = s_list[par][2];
+            const int* cnt = s_cnt[par][2];
+            const int n = total(cnt);
+            for (int i = tid; i < n; i += GRAD_THREADS) {
+                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                const int v = (x - 2) + VX * (y - 2);
+                T sx = T(0), sy = T(0), sz = T(0), ws = T(0);
+#pragma unroll
+                for (int qq = 1; qq < 19; ++qq) {
+                    const bool f = tag_fluid(tl[1 + EZ[qq]][e + EX[qq] + WX * EY[qq]]);
+                    const T* c = cl[1 + EZ[qq]];
+                    const int o = v + EX[qq] + VX * EY[qq];
+                    sx += keep_if(f, c[o]) * wq<T>(qq);
+                    sy += keep_if(f, c[NV + o]) * wq<T>(qq);
+                    sz += keep_if(f, c[2 * NV + o]) * wq<T>(qq);
+                    ws += keep_if(f, wq<T>(qq));
+                }
+                cn[v] = sx / ws; cn[NV + v] = sy / ws; cn[2 * NV + v] = sz / ws;
+            }
+        }
+        lds_barrier();
+
+        // K7: curvature and the force record of the owned fluid nodes of plane q
+        // (src/main_iteration_GPU.cu:908-1003)
+        if (p - 4 >= k0) {
+            const int q = p - 4;
+            const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
+            const T* cno = s_cnorm[s3(q)];
+            const int16_t* list = s_list[par][3];
+            const int* cnt = s_cnt[par][3];
+            const int n = total(cnt);
+            static_assert(NO <= GRAD_THREADS, "one owned node per thread");
+            if (tid < n) {
+                const int e = item(list, cnt, tid), x = e % WX, y = e / WX;
+                const int v = (x - 2) + VX * (y - 2);
+                auto comp = [&](int c) {
+                    return [&, c](int dx, int dy, int dz) -> T { return cl[1 + dz][c * NV + v + dx + VX * dy]; };
+                };
+                const T curv = curvature_of<T>(comp(0), comp(1), comp(2));
+                T* a = A.aux + 4 * (int64_t)rec;
+                a[0] = cl[1][v];
+                a[1] = cl[1][NV + v];
+                a[2] = cl[1][2 * NV + v];
+                a[3] = T(0.5) * P.gamma * curv * cno[(x - H) + TX * (y - H)];
+            }
+        }
+    }
+}
+
+}  // namespace dev
+}  // namespace mflbm

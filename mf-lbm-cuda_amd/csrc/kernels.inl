@@ -344,23 +344,100 @@ __global__ __launch_bounds__(256) void k_extra_cells(Sparse<T> S, T* __restrict_
 // Colour-gradient chain
 // ---------------------------------------------------------------------------------------
 // ISO4 central-difference stencil along one axis, summed in the reference order
-// (src/main_iteration_GPU.cu:765-791, 914-996).
-template <typename T, int AX>
-__device__ __forceinline__ T iso4(const T* __restrict__ a, int c, int sx, int sy, int sz) {
+// (src/main_iteration_GPU.cu:765-791, 914-996).  `a(dx, dy, dz)` returns the value at the
+// neighbour offset; the dense-array and the LDS-tile callers share this one body so they
+// round identically.
+template <typename T, int AX, class A>
+__device__ __forceinline__ T iso4_at(A&& a) {
     // offsets of the 4 "plus" terms of the second group, per axis
     constexpr int G2[3][4][3] = {{{1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {1, 0, -1}},
                                  {{1, 1, 0}, {-1, 1, 0}, {0, 1, 1}, {0, 1, -1}},
                                  {{1, 0, 1}, {-1, 0, 1}, {0, 1, 1}, {0, -1, 1}}};
-    const int o1 = AX == 0 ? sx : (AX == 1 ? sy : sz);
-    const T d1 = a[c + o1] - a[c - o1];
+    constexpr int ax = AX == 0, ay = AX == 1, az = AX == 2;
+    const T d1 = a(ax, ay, az) - a(-ax, -ay, -az);
     T acc = T(0);
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        const int o = G2[AX][m][0] * sx + G2[AX][m][1] * sy + G2[AX][m][2] * sz;
-        acc = (m == 0) ? a[c + o] : acc + a[c + o];
-        acc = acc - a[c - o];
+        const int ox = G2[AX][m][0], oy = G2[AX][m][1], oz = G2[AX][m][2];
+        acc = (m == 0) ? a(ox, oy, oz) : acc + a(ox, oy, oz);
+        acc = acc - a(-ox, -oy, -oz);
     }
     return (T(1.0) / T(6.0)) * d1 + (T(1.0) / T(12.0)) * acc;
+}
+
+template <typename T, int AX>
+__device__ __forceinline__ T iso4(const T* __restrict__ a, int c, int sx, int sy, int sz) {
+    return iso4_at<T, AX>([&](int dx, int dy, int dz) -> T { return a[c + dx * sx + dy * sy + dz * sz]; });
+}
+
+// Interface normal of a fluid node from its ISO4 colour gradient (cx, cy, cz), with the
+// geometrical wetting correction at fluid-boundary nodes (wetting = walls_type -1) against
+// the solid normal (nwx, nwy, nwz): src/main_iteration_GPU.cu:793-807 and 809-878.
+// Returns c_norm; (vx, vy, vz) receive the unit normal (zero where |grad| < 1e-6).
+template <typename T, class SolidNormal>
+__device__ __forceinline__ T interface_normal(T cx, T cy, T cz, bool wetting, T ct, SolidNormal&& solid, T& vx, T& vy,
+                                              T& vz) {
+    const T cn = sqrt(cx * cx + cy * cy + cz * cz);
+    if (cn < T(1e-6)) {
+        vx = T(0); vy = T(0); vz = T(0);
+        return T(0);
+    }
+    T vx0 = cx / cn, vy0 = cy / cn, vz0 = cz / cn;
+    if (wetting && cn > T(1e-6)) {
+        const T lambda = T(0.5), leps = T(1e-6);
+        T nwx, nwy, nwz;
+        solid(nwx, nwy, nwz);
+        T vx1 = vx0 - lambda * (vx0 + nwx);
+        T vy1 = vy0 - lambda * (vy0 + nwy);
+        T vz1 = vz0 - lambda * (vz0 + nwz);
+        T e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - ct;
+        if ((fabs(vx0 + nwx) + fabs(vy0 + nwy) + fabs(vz0 + nwz) > leps ||
+             fabs(vx0 - nwx) + fabs(vy0 - nwy) + fabs(vz0 - nwz) > leps) && e0 > leps) {
+            T e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
+            T tt = T(1.0) / (e1 - e0);
+            T vx2 = tt * (vx0 * e1 - vx1 * e0);
+            T vy2 = tt * (vy0 * e1 - vy1 * e0);
+            T vz2 = tt * (vz0 * e1 - vz1 * e0);
+            T e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
+            if (e2 > leps) {
+                for (int it = 2; it <= 4; ++it) {
+                    vx0 = vx1; vy0 = vy1; vz0 = vz1;
+                    vx1 = vx2; vy1 = vy2; vz1 = vz2;
+                    e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - sqrt(vx0 * vx0 + vy0 * vy0 + vz0 * vz0) * ct;
+                    e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
+                    tt = T(1.0) / (e1 - e0);
+                    if (isinf(tt)) break;
+                    vx2 = tt * (vx0 * e1 - vx1 * e0);
+                    vy2 = tt * (vy0 * e1 - vy1 * e0);
+                    vz2 = tt * (vz0 * e1 - vz1 * e0);
+                    e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
+                }
+            }
+            tt = T(1.0) / (T(1e-30) + sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2));
+            vx0 = vx2 * tt; vy0 = vy2 * tt; vz0 = vz2 * tt;
+        }
+    }
+    vx = vx0; vy = vy0; vz = vz0;
+    return cn;
+}
+
+// Curvature of the interface from the ISO4 derivatives of the unit normal field
+// (src/main_iteration_GPU.cu:914-1001).  `nx/ny/nz(dx, dy, dz)` read the normal
+// components at a neighbour offset.
+template <typename T, class NX, class NY, class NZ>
+__device__ __forceinline__ T curvature_of(NX&& nx, NY&& ny, NZ&& nz) {
+    const T kxx = iso4_at<T, 0>(nx);
+    const T kyy = iso4_at<T, 1>(ny);
+    const T kzz = iso4_at<T, 2>(nz);
+    const T kxy = iso4_at<T, 1>(nx);
+    const T kxz = iso4_at<T, 2>(nx);
+    const T kyx = iso4_at<T, 0>(ny);
+    const T kyz = iso4_at<T, 2>(ny);
+    const T kzx = iso4_at<T, 0>(nz);
+    const T kzy = iso4_at<T, 1>(nz);
+    const T cx = nx(0, 0, 0), cy = ny(0, 0, 0), cz = nz(0, 0, 0);
+    return (cx * cx - T(1.0)) * kxx + (cy * cy - T(1.0)) * kyy + (cz * cz - T(1.0)) * kzz + cx * cy * (kxy + kyx) +
+           cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
 }
 
 // K3: phi at solid-boundary nodes = D3Q19-weighted mean over fluid neighbours, [-2, n+3].
@@ -400,48 +477,11 @@ __global__ __launch_bounds__(256) void k_normals(Fields<T> F, StepConst<T> P, co
     const T cx = iso4<T, 0>(F.phi, c4, sx, sy, sz);
     const T cy = iso4<T, 1>(F.phi, c4, sx, sy, sz);
     const T cz = iso4<T, 2>(F.phi, c4, sx, sy, sz);
-    const T cn = sqrt(cx * cx + cy * cy + cz * cz);
-    if (cn < T(1e-6)) {
-        F.cnx[c2] = T(0); F.cny[c2] = T(0); F.cnz[c2] = T(0); F.cnorm[c2] = T(0);
-        return;
-    }
-    T vx0 = cx / cn, vy0 = cy / cn, vz0 = cz / cn;
-    F.cnorm[c2] = cn;
-    if (F.wtype[c4] == -1 && cn > T(1e-6)) {
-        // src/main_iteration_GPU.cu:809-878
-        const T lambda = T(0.5), leps = T(1e-6), ct = P.cos_theta;
-        const T nwx = F.snx[c4], nwy = F.sny[c4], nwz = F.snz[c4];
-        T vx1 = vx0 - lambda * (vx0 + nwx);
-        T vy1 = vy0 - lambda * (vy0 + nwy);
-        T vz1 = vz0 - lambda * (vz0 + nwz);
-        T e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - ct;
-        if ((fabs(vx0 + nwx) + fabs(vy0 + nwy) + fabs(vz0 + nwz) > leps ||
-             fabs(vx0 - nwx) + fabs(vy0 - nwy) + fabs(vz0 - nwz) > leps) && e0 > leps) {
-            T e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
-            T tt = T(1.0) / (e1 - e0);
-            T vx2 = tt * (vx0 * e1 - vx1 * e0);
-            T vy2 = tt * (vy0 * e1 - vy1 * e0);
-            T vz2 = tt * (vz0 * e1 - vz1 * e0);
-            T e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
-            if (e2 > leps) {
-                for (int it = 2; it <= 4; ++it) {
-                    vx0 = vx1; vy0 = vy1; vz0 = vz1;
-                    vx1 = vx2; vy1 = vy2; vz1 = vz2;
-                    e0 = (nwx * vx0 + nwy * vy0 + nwz * vz0) - sqrt(vx0 * vx0 + vy0 * vy0 + vz0 * vz0) * ct;
-                    e1 = (nwx * vx1 + nwy * vy1 + nwz * vz1) - sqrt(vx1 * vx1 + vy1 * vy1 + vz1 * vz1) * ct;
-                    tt = T(1.0) / (e1 - e0);
-                    if (isinf(tt)) break;
-                    vx2 = tt * (vx0 * e1 - vx1 * e0);
-                    vy2 = tt * (vy0 * e1 - vy1 * e0);
-                    vz2 = tt * (vz0 * e1 - vz1 * e0);
-                    e2 = (nwx * vx2 + nwy * vy2 + nwz * vz2) - sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2) * ct;
-                }
-            }
-            tt = T(1.0) / (T(1e-30) + sqrt(vx2 * vx2 + vy2 * vy2 + vz2 * vz2));
-            vx0 = vx2 * tt; vy0 = vy2 * tt; vz0 = vz2 * tt;
-        }
-    }
-    F.cnx[c2] = vx0; F.cny[c2] = vy0; F.cnz[c2] = vz0;
+    T vx, vy, vz;
+    F.cnorm[c2] = interface_normal<T>(cx, cy, cz, F.wtype[c4] == -1, P.cos_theta,
+                                      [&](T& nwx, T& nwy, T& nwz) { nwx = F.snx[c4]; nwy = F.sny[c4]; nwz = F.snz[c4]; },
+                                      vx, vy, vz);
+    F.cnx[c2] = vx; F.cny[c2] = vy; F.cnz[c2] = vz;
 }
 
 // K6: normals at solid-boundary nodes in [0, n+1] from fluid neighbours.
@@ -474,19 +514,11 @@ __global__ __launch_bounds__(256) void k_normal_extrap(Fields<T> F, const uint32
 template <typename T>
 __device__ __forceinline__ T curvature_at(const Fields<T>& F, int c2) {
     const Dims& d = F.d;
-    const int sx = 1, sy = d.X2, sz = d.X2 * d.Y2;
-    const T kxx = iso4<T, 0>(F.cnx, c2, sx, sy, sz);
-    const T kyy = iso4<T, 1>(F.cny, c2, sx, sy, sz);
-    const T kzz = iso4<T, 2>(F.cnz, c2, sx, sy, sz);
-    const T kxy = iso4<T, 1>(F.cnx, c2, sx, sy, sz);
-    const T kxz = iso4<T, 2>(F.cnx, c2, sx, sy, sz);
-    const T kyx = iso4<T, 0>(F.cny, c2, sx, sy, sz);
-    const T kyz = iso4<T, 2>(F.cny, c2, sx, sy, sz);
-    const T kzx = iso4<T, 0>(F.cnz, c2, sx, sy, sz);
-    const T kzy = iso4<T, 1>(F.cnz, c2, sx, sy, sz);
-    const T cx = F.cnx[c2], cy = F.cny[c2], cz = F.cnz[c2];
-    return (cx * cx - T(1.0)) * kxx + (cy * cy - T(1.0)) * kyy + (cz * cz - T(1.0)) * kzz + cx * cy * (kxy + kyx) +
-           cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+    const int sy = d.X2, sz = d.X2 * d.Y2;
+    auto at = [&](const T* a) {
+        return [=](int dx, int dy, int dz) -> T { return a[c2 + dx + dy * sy + dz * sz]; };
+    };
+    return curvature_of<T>(at(F.cnx), at(F.cny), at(F.cnz));
 }
 
 template <typename T>

@@ -94,6 +94,52 @@ struct Sparse {
     T* aux;                         // 4 * nf force records (cn_x, cn_y, cn_z, 0.5 gamma curv c_norm)
 };
 
+// Inputs of the fused gradient-chain kernel (gradient.inl).  `tag` holds, for every node of
+// the four-ghost-layer grid, (t << 2) | (walls_type & 3) with walls_type 0 -> 0, 1 -> 1,
+// 2 -> 2, -1 -> 3, and t the fluid-list index (force-record index) of an interior fluid
+// node (0 elsewhere).
+template <typename T>
+struct GradArgs {
+    Dims d;
+    T* phi;                           // S4
+    const int32_t* __restrict__ tag;  // S4
+    const T* __restrict__ snx;        // S4 solid normals, read at fluid-boundary nodes
+    const T* __restrict__ sny;
+    const T* __restrict__ snz;
+    T* aux;                           // 4 * nf force records
+    int shape, tiles_x, tiles_y, kchunk;
+};
+
+// Workgroup shapes of the fused gradient kernel: owned nodes in x and y, threads.
+struct GradShape {
+    int tx, ty, threads;
+};
+constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}};
+constexpr int GRAD_NSHAPES = sizeof(GRAD_SHAPES) / sizeof(GRAD_SHAPES[0]);
+constexpr int GRAD_SHAPE_DEFAULT = 0;
+
+// Work lists of the fused gradient kernel are built per wave: wave w appends to segment
+// w (capacity seg) and records its count cnt[w].  Item i of the concatenated list lies in
+// the last segment whose running prefix does not exceed i.  Returns the entry, or -1 for
+// i outside the list.
+template <int WAVES>
+MFLBM_HD int list_item(const int16_t* list, const int* cnt, int seg, int i) {
+    int sg = 0, base = 0, acc = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES - 1; ++w) {
+        acc += cnt[w];
+        const bool past = i >= acc;
+        sg = past ? w + 1 : sg;
+        base = past ? acc : base;
+    }
+    const int k = i - base;
+    if (i < 0 || k >= cnt[sg]) return -1;
+    return list[sg * seg + k];
+}
+
+constexpr int TAG_FLUID = 0, TAG_SOLID = 1, TAG_SOLID_BND = 2, TAG_WET = 3;
+MFLBM_HD bool tag_fluid(int code) { return code == TAG_FLUID || code == TAG_WET; }
+
 struct BoundaryFlags {
     int kper, jper;
     int open_z;      // kper == 0 and no z walls

@@ -17,6 +17,7 @@ struct Launch {
     static void normals(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, hipStream_t s);
     static void normal_extrap(const Fields<T>& F, const NodeLists& L, hipStream_t s);
     static void curvature(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s);
+    static void gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_t s);
     static void aux_refresh(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s);
     static void cells(const Sparse<T>& S, T* dense, const Dims& d, const int32_t* ext_node, const int32_t* ext_off,
                       int n_ext, bool to_dense, hipStream_t s);

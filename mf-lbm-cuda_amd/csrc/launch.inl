@@ -1,6 +1,7 @@
 // Definitions of the Launch<T> members; included by kernels_f64.hip and kernels_f32.hip.
 #pragma once
 #include "kernels.inl"
+#include "gradient.inl"
 #include "launch.h"
 
 namespace mflbm {
@@ -59,6 +60,25 @@ void Launch<T>::normal_extrap(const Fields<T>& F, const NodeLists& L, hipStream_
 template <typename T>
 void Launch<T>::curvature(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s) {
     if (L.n_fluid > 0) dev::k_curvature<T><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid, aux);
+}
+
+template <typename T>
+void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_t s) {
+    const int chunks = (A.d.nz + A.kchunk - 1) / A.kchunk;
+    const unsigned grid = (unsigned)(A.tiles_x * A.tiles_y * chunks);
+    static_assert(GRAD_NSHAPES == 3, "one case per shape");
+    switch (A.shape) {
+#define MFLBM_GRAD_SHAPE(n)                                                                                  \
+    case n:                                                                                                  \
+        dev::k_gradient<T, GRAD_SHAPES[n].tx, GRAD_SHAPES[n].ty, GRAD_SHAPES[n].threads>                     \
+            <<<grid, GRAD_SHAPES[n].threads, 0, s>>>(A, P);                                                  \
+        break;
+        MFLBM_GRAD_SHAPE(0)
+        MFLBM_GRAD_SHAPE(1)
+        MFLBM_GRAD_SHAPE(2)
+#undef MFLBM_GRAD_SHAPE
+        default: break;
+    }
 }
 
 template <typename T>

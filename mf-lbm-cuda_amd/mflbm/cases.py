@@ -73,13 +73,14 @@ def config1(seed=1):
     return geo, ctrl
 
 
-def config2(seed=2, n=(256, 256, 256)):
-    """256^3 drainage: spheres r=8, porosity 0.35, theta 30, viscosity ratio 0.1."""
+def config2(seed=2, n=(256, 256, 256), geometry=True):
+    """256^3 drainage: spheres r=8, porosity 0.35, theta 30, viscosity ratio 0.1.
+    ``geometry=False`` returns (None, control) without building the sphere pack."""
     nu2 = nu_from_tau(0.7)
     nu1 = 0.1 * nu2  # fluid 1 (invading) viscosity = 0.1 x defending fluid viscosity
     ctrl = drainage_control(n=n, theta=30.0, nu1=nu1, nu2=nu2, gamma=0.01, body_force=2e-4,
                             interface_z0=8.0)
-    geo = sphere_geometry(n, 8.0, 0.35, seed, buffer=8)
+    geo = sphere_geometry(n, 8.0, 0.35, seed, buffer=8) if geometry else None
     return geo, ctrl
 
 

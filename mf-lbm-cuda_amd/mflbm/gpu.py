@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("lbm_gamma", C.c_double), ("force_z", C.c_double), ("la_nui1", C.c_double), ("la_nui2", C.c_double),
         ("lbm_beta", C.c_double), ("phi_inlet", C.c_double), ("sa_inject", C.c_double), ("cos_theta", C.c_double),
         ("rho_in", C.c_double), ("rho_out", C.c_double), ("relaxation", C.c_double),
-        ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("precision", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -48,8 +48,7 @@ class MonitorOut(C.Structure):
 
 
 class KernelTimes(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("collide_odd_ms", "collide_even_ms", "boundary_ms", "phi_extrap_ms",
-                                          "normals_ms", "normal_extrap_ms", "curvature_ms")] + [
+    _fields_ = [(n, C.c_double) for n in ("collide_odd_ms", "collide_even_ms", "boundary_ms", "gradient_ms")] + [
         ("n_odd", C.c_int32), ("n_even", C.c_int32)]
 
 
@@ -99,13 +98,16 @@ def load():
     return L
 
 
-def make_params(p: dict, precision=FP64) -> Params:
+FLAG_SPLIT_GRADIENT = 1  # MFLBM_FLAG_SPLIT_GRADIENT
+
+
+def make_params(p: dict, precision=FP64, flags=0) -> Params:
     cp = Params()
     for name, _ in Params._fields_:
         if name == "precision":
             cp.precision = precision
-        elif name == "reserved":
-            cp.reserved = 0
+        elif name == "flags":
+            cp.flags = flags
         else:
             setattr(cp, name, p[name])
     return cp
@@ -118,7 +120,7 @@ def _dp(a):
 class Solver:
     """One device-resident simulation (handle of the C ABI)."""
 
-    def __init__(self, params: dict, geometry: np.ndarray, precision=FP64):
+    def __init__(self, params: dict, geometry: np.ndarray, precision=FP64, split_gradient=False):
         L = load()
         self.p = dict(params)
         self.precision = precision
@@ -127,7 +129,7 @@ class Solver:
         geo = np.ascontiguousarray(geometry, dtype=np.int8)
         if geo.shape != (self.nz, self.ny, self.nx):
             raise ValueError(f"geometry shape {geo.shape} != {(self.nz, self.ny, self.nx)}")
-        self._cp = make_params(self.p, precision)
+        self._cp = make_params(self.p, precision, FLAG_SPLIT_GRADIENT if split_gradient else 0)
         h = C.c_void_p()
         rc = L.mflbm_create(C.byref(self._cp), geo.ctypes.data_as(C.POINTER(C.c_int8)), C.byref(h))
         self.h = h

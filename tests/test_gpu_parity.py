@@ -80,6 +80,52 @@ def test_setup_and_steps_bitwise(name):
     compare_state(o, g, tag=f"{name} step40")
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_gradient_matches_split(name):
+    """The fused gradient-chain kernel and the four split kernels give identical steps.
+
+    No download between the steps, so every collision reads force records written by
+    the kernel under test."""
+    geo, p = CASES[name]()
+    say(f"case={name} fused vs split gradient, 30 steps")
+    a = gpu.Solver(p, geo, gpu.FP64)
+    b = gpu.Solver(p, geo, gpu.FP64, split_gradient=True)
+    a.setup()
+    b.setup()
+    a.run(1, 30)
+    b.run(1, 30)
+    assert_same(a.download("pdf"), b.download("pdf"), what=f"{name} pdf")
+    for n in ("phi", "cn_x", "cn_y", "cn_z", "c_norm", "curv"):
+        assert_same(a.download(n), b.download(n), what=f"{name} {n}")
+
+
+def test_fused_gradient_short_z_runs(monkeypatch):
+    """Workgroups marching 5-plane z-runs (many run boundaries) still match the oracle."""
+    monkeypatch.setenv("MFLBM_GRAD_KCHUNK", "5")
+    geo, p = CASES["drainage_ywalls"]()
+    say("case=drainage_ywalls kchunk=5, 24 steps")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    o.run(1, 24)
+    g.run(1, 24)
+    compare_state(o, g, tag="kchunk5 step24")
+
+
+def test_fused_gradient_fp32_matches_split():
+    geo, p = CASES["drainage_yper"]()
+    say("case=drainage_yper fp32 fused vs split, 30 steps")
+    a = gpu.Solver(p, geo, gpu.FP32)
+    b = gpu.Solver(p, geo, gpu.FP32, split_gradient=True)
+    a.setup()
+    b.setup()
+    a.run(1, 30)
+    b.run(1, 30)
+    # FP32 is built with contraction allowed, so the two paths may round differently
+    assert_same(a.download("pdf"), b.download("pdf"), 1e-5, what="fp32 pdf")
+    assert_same(a.download("phi"), b.download("phi"), 1e-5, what="fp32 phi")
+
+
 def test_config1_1000_steps():
     """64^3 drainage of the first workload, 1000 steps, relative L-inf <= 1e-12 (bitwise expected)."""
     geo, ctrl = cases.config1()
