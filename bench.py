@@ -79,6 +79,8 @@ PMC_STEPS = 4  # steps 1..4: two odd and two even collisions
 
 def kernel_group(name):
     """Short name of a step kernel from its demangled name (None for other kernels)."""
+    if "k_probe_copy_blocked" in name:
+        return "probe_blocked"
     if PROBE_KERNEL in name:
         return "probe"
     for key, short in (("k_collide<double, true>", "collide_odd"), ("k_collide<float, true>", "collide_odd"),
@@ -105,15 +107,16 @@ def pmc_child(args):
     print("pmc child done", flush=True)
 
 
-def counter_pass(counter, geo_path, args, timeout=240):
-    """Counter values (KiB per dispatch) of one rocprofv3 --pmc pass, grouped by kernel."""
+def counter_pass(counters, geo_path, args, timeout=240):
+    """Values of the given counters per dispatch from one rocprofv3 --pmc pass, grouped by
+    kernel: {group: {counter: [value per dispatch]}}."""
     rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     out = tempfile.mkdtemp(prefix="mflbm_pmc_", dir="/tmp")
-    cmd = [rp, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+    cmd = [rp, "--pmc", *counters, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
            sys.executable, os.path.abspath(__file__), "--pmc-child", geo_path, "--precision", args.precision,
            "--probe-mib", str(args.probe_mib)]
     env = dict(os.environ, TMPDIR="/tmp")
-    log(f"rocprofv3 --pmc {counter} pass")
+    log(f"rocprofv3 --pmc {' '.join(counters)}")
     proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                             start_new_session=True, text=True)
     try:
@@ -122,48 +125,69 @@ def counter_pass(counter, geo_path, args, timeout=240):
         os.killpg(proc.pid, signal.SIGKILL)
         proc.communicate()
         shutil.rmtree(out, ignore_errors=True)
-        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+        raise RuntimeError(f"rocprofv3 --pmc {counters} timed out")
     # rocprofv3 itself may end with SIGSEGV after a complete run; the child's last line and
     # the result file decide
     files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
     if "pmc child done" not in text or not files:
         shutil.rmtree(out, ignore_errors=True)
-        raise RuntimeError(f"rocprofv3 --pmc {counter} failed (rc {proc.returncode}): {text[-400:]}")
+        raise RuntimeError(f"rocprofv3 --pmc {counters} failed (rc {proc.returncode}): {text[-400:]}")
     vals = {}
     with open(files[0], newline="") as fh:
         for row in csv.DictReader(fh):
-            if row["Counter_Name"] != counter:
-                continue
             g = kernel_group(row["Kernel_Name"])
             if g:
-                vals.setdefault(g, []).append(float(row["Counter_Value"]))
+                vals.setdefault(g, {}).setdefault(row["Counter_Name"], []).append(float(row["Counter_Value"]))
     shutil.rmtree(out, ignore_errors=True)
     return vals
 
 
+GROUPS = ("collide_odd", "collide_even", "gradient", "probe", "probe_blocked")
+
+
 def measure_traffic(geo, args):
-    """HBM bytes per dispatch of the step kernels from FETCH_SIZE and WRITE_SIZE passes,
-    each scaled so that the plain copy of known size reads and writes exactly its bytes."""
+    """HBM bytes per dispatch of the step kernels.
+
+    Two measures, each from rocprofv3 --pmc passes over a child running the same workload:
+    * `requests`: the L2-to-memory read requests by size (32, 64, 128 B) and write requests
+      (32, 64 B), summed over all L2 channels, times their sizes;
+    * `fetch_write`: the FETCH_SIZE and WRITE_SIZE derived counters, scaled so that the
+      grid-stride plain copy of known size reads and writes exactly its size.
+    Both plain copies are reported so that each measure can be checked against a known size."""
     tmp = tempfile.mkdtemp(prefix="mflbm_geo_", dir="/tmp")
     geo_path = os.path.join(tmp, "geo.npy")
     np.save(geo_path, geo)
     try:
-        f = counter_pass("FETCH_SIZE", geo_path, args)
-        w = counter_pass("WRITE_SIZE", geo_path, args)
+        rd = counter_pass(["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum",
+                           "TCC_EA0_RDREQ_sum"], geo_path, args)
+        wr = counter_pass(["TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum"], geo_path, args)
+        f = counter_pass(["FETCH_SIZE"], geo_path, args)
+        w = counter_pass(["WRITE_SIZE"], geo_path, args)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+    med = lambda v: float(np.median(v))
     nbytes = args.probe_mib << 20
-    sf = nbytes / (f["probe"][-1] * 1024.0)
-    sw = nbytes / (w["probe"][-1] * 1024.0)
-    res = {"fetch_scale": round(sf, 4), "write_scale": round(sw, 4)}
-    for g in ("collide_odd", "collide_even", "gradient"):
+    res = {"copy_bytes": nbytes, "requests": {}, "fetch_write": {}}
+    for g in GROUPS:
+        if g in rd and g in wr:
+            r = rd[g]
+            rb = 32 * med(r["TCC_EA0_RDREQ_32B_sum"]) + 64 * med(r["TCC_EA0_RDREQ_64B_sum"]) + \
+                128 * med(r["TCC_EA0_RDREQ_128B_sum"])
+            w64 = med(wr[g]["TCC_EA0_WRREQ_64B_sum"])
+            wb = 64 * w64 + 32 * (med(wr[g]["TCC_EA0_WRREQ_sum"]) - w64)
+            res["requests"][g] = {"fetch": round(rb), "write": round(wb), "total": round(rb + wb)}
+    sf = nbytes / (med(f["probe"]["FETCH_SIZE"]) * 1024.0)
+    sw = nbytes / (med(w["probe"]["WRITE_SIZE"]) * 1024.0)
+    res["fetch_write"]["scale"] = {"fetch": round(sf, 4), "write": round(sw, 4)}
+    for g in GROUPS:
         if g in f and g in w:
-            fb = float(np.median(f[g])) * 1024.0 * sf
-            wb = float(np.median(w[g])) * 1024.0 * sw
-            res[g] = {"fetch": round(fb), "write": round(wb), "total": round(fb + wb)}
-    log("traffic per dispatch (GB): " + ", ".join(f"{g} {res[g]['total'] / 1e9:.3f}" for g in
-                                                  ("collide_odd", "collide_even", "gradient") if g in res)
-        + f"; counter scales fetch x{sf:.3f} write x{sw:.3f}")
+            fb = med(f[g]["FETCH_SIZE"]) * 1024.0 * sf
+            wb = med(w[g]["WRITE_SIZE"]) * 1024.0 * sw
+            res["fetch_write"][g] = {"fetch": round(fb), "write": round(wb), "total": round(fb + wb)}
+    for kind in ("requests", "fetch_write"):
+        d = res[kind]
+        log(f"traffic per dispatch ({kind}, GB): " + ", ".join(
+            f"{g} {d[g]['fetch'] / 1e9:.3f}+{d[g]['write'] / 1e9:.3f}" for g in GROUPS if g in d))
     return res
 
 
@@ -240,6 +264,22 @@ def main():
     log(f"collide odd {odd_ms:.3f} ms, even {even_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), "
         f"boundary {bnd_ms:.3f} ms, gradient {grad_ms:.3f} ms, step (kernels) {step_ms:.3f} ms")
 
+    # the same kernels on a colour field with no uniform region (random phi in [-1, 1] on
+    # every node, populations re-initialised from it): the gradient chain's worst case,
+    # where no plane takes the exact uniform-region shortcut
+    rng = np.random.default_rng(1)
+    sim.upload("phi", rng.uniform(-1.0, 1.0, size=sim.shape(4)))
+    sim.init_equilibrium()
+    sim.color_gradient()
+    sim.run(1, 4)
+    km = sim.profile(5, args.profile_steps)
+    nm = max(1, km["n_odd"] + km["n_even"])
+    mixed = {"collide_odd": round(km["collide_odd_ms"] / max(1, km["n_odd"]), 4),
+             "collide_even": round(km["collide_even_ms"] / max(1, km["n_even"]), 4),
+             "boundary": round(km["boundary_ms"] / nm, 4), "gradient": round(km["gradient_ms"] / nm, 4)}
+    log(f"mixed colour field: gradient {mixed['gradient']:.3f} ms, collide odd {mixed['collide_odd']:.3f} / "
+        f"even {mixed['collide_even']:.3f} ms")
+
     probe = None
     if args.probe_mib > 0:
         nbytes = args.probe_mib << 20
@@ -255,8 +295,9 @@ def main():
         log(f"CPU baseline {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
 
     coll_traffic = None
-    if traffic and "collide_odd" in traffic and "collide_even" in traffic:
-        coll_traffic = round(0.5 * (traffic["collide_odd"]["total"] + traffic["collide_even"]["total"]))
+    if traffic and "collide_odd" in traffic["requests"] and "collide_even" in traffic["requests"]:
+        t = traffic["requests"]
+        coll_traffic = round(0.5 * (t["collide_odd"]["total"] + t["collide_even"]["total"]))
     out = {
         "metric": "MLUPS", "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True,
@@ -273,6 +314,7 @@ def main():
         "step_roofline_frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {"collide_odd": round(odd_ms, 4), "collide_even": round(even_ms, 4),
                       "boundary": round(bnd_ms, 4), "gradient": round(grad_ms, 4), "step": round(step_ms, 4)},
+        "kernel_ms_mixed_field": mixed,
         "traffic_bytes": traffic,
         "copy_probe": probe,
         "cpu_baseline": cpu,

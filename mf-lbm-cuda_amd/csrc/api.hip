@@ -272,8 +272,22 @@ void invalidate_graphs(mflbm_handle* h) {
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
 }
 
+// Plain copies for the bandwidth probe: a grid-stride loop, and one-shot blocks that each
+// move 4 x 256 x 32 B with all loads issued before the stores.
 __global__ __launch_bounds__(256) void k_probe_copy(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) dst[t] = src[t];
+}
+
+__global__ __launch_bounds__(256) void k_probe_copy_blocked(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                            int64_t n) {
+    const int64_t b = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    double4 v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        if (b + m * 256 < n) v[m] = src[b + m * 256];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        if (b + m * 256 < n) dst[b + m * 256] = v[m];
 }
 
 }  // namespace
@@ -658,14 +672,23 @@ int mflbm_bandwidth_probe(mflbm_handle* h, int64_t bytes, int32_t repeats, doubl
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const unsigned grid = 256 * 8 * 4;
-    k_probe_copy<<<grid, 256, 0, h->stream>>>(a, b, n);  // warm-up
-    (void)hipEventRecord(e0, h->stream);
-    for (int r = 0; r < repeats; ++r) k_probe_copy<<<grid, 256, 0, h->stream>>>(a, b, n);
-    (void)hipEventRecord(e1, h->stream);
-    e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    // both copy shapes; the faster one is the reported plain-copy rate
+    const unsigned grid = 256 * 8 * 4, grid_b = (unsigned)((n + 1023) / 1024);
+    float ms = 0.f, best = 0.f;
+    for (int variant = 0; variant < 2 && e == hipSuccess; ++variant) {
+        auto launch = [&]() {
+            if (variant == 0) k_probe_copy<<<grid, 256, 0, h->stream>>>(a, b, n);
+            else k_probe_copy_blocked<<<grid_b, 256, 0, h->stream>>>(a, b, n);
+        };
+        launch();  // warm-up
+        (void)hipEventRecord(e0, h->stream);
+        for (int r = 0; r < repeats; ++r) launch();
+        (void)hipEventRecord(e1, h->stream);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && (variant == 0 || ms < best)) best = ms;
+    }
+    ms = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);

@@ -99,6 +99,28 @@ def test_fused_gradient_matches_split(name):
         assert_same(a.download(n), b.download(n), what=f"{name} {n}")
 
 
+def test_mixed_phase_field_bitwise():
+    """A random colour field (no uniform region anywhere) against the oracle: every
+    gradient stage takes its full computation, none the uniform-region shortcut."""
+    geo, p = CASES["drainage_ywalls"]()
+    say("case=drainage_ywalls random phi, 20 steps")
+    o, g = pair(geo, p)
+    o.init_phi(1, p["interface_z0"])
+    g.init_phi(1, p["interface_z0"])
+    rng = np.random.default_rng(12)
+    ph = rng.uniform(-1.0, 1.0, size=o.shape(4))
+    o.array("phi")[...] = ph
+    g.upload("phi", ph)
+    o.init_pdf()
+    g.init_equilibrium()
+    o.color_gradient()
+    g.color_gradient()
+    compare_state(o, g, tag="random phi init")
+    o.run(1, 20)
+    g.run(1, 20)
+    compare_state(o, g, tag="random phi step20")
+
+
 def test_fused_gradient_short_z_runs(monkeypatch):
     """Workgroups marching 5-plane z-runs (many run boundaries) still match the oracle."""
     monkeypatch.setenv("MFLBM_GRAD_KCHUNK", "5")
