@@ -26,3 +26,13 @@ def test_struct_sizes_match_header_layout():
     # mflbm_params: 3 int64 + 11 int32 (+4 pad) + 11 double + 2 int32
     assert ctypes.sizeof(gpu.Params) == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 2 * 4
     assert ctypes.sizeof(gpu.MonitorOut) == 7 * 8 + 3 * 8 + 8
+
+
+def test_network_header_matches_library_and_bindings():
+    from mflbm import nn
+    with open(os.path.join(os.path.dirname(gpu.HEADER), "mfnn.h")) as fh:
+        declared = sorted(set(re.findall(r"\b(mfnn_[a-z_0-9]+)\s*\(", fh.read())))
+    lib = ctypes.CDLL(gpu.LIB_PATH)
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == sorted(nn.SIGNATURES)
