@@ -99,6 +99,13 @@ int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0);
  * This is the warm-start hook. */
 int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w);
 
+/* As mflbm_init_equilibrium, with the constant terms at the collision's own rest state,
+ * rho * (4/21, 13/126, 1/63) (MRT e = -11 rho, e2 = 0), in place of the D3Q19 weights
+ * (1/3, 1/18, 1/36) whose fourth-order moment e2 = 3 rho the first collision discards.
+ * No counterpart in the reference: it is the initialisation of the warm start, which
+ * removes the start-up mass pulse the D3Q19 weights leave in the bounce-back links. */
+int mflbm_init_relaxed(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w);
+
 /* Colour-gradient chain on the current phi: the reference's host color_gradient()
  * (src/Phase_gradient.cpp:15-291) that precedes the first step (src/main.cpp:92). */
 int mflbm_color_gradient(mflbm_handle* h);

@@ -61,6 +61,7 @@ struct mflbm_handle {
     int mon_nsplit = 0;
     // captured step graphs: [0] even step, [1] odd step
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double rho_ref = 1.0;  // FP32 storage reference density (ShiftedStorage)
     std::string err;
 };
 
@@ -104,6 +105,7 @@ StepConst<T> consts(const mflbm_handle* h) {
     c.gamma = (T)p.lbm_gamma; c.force_z = (T)p.force_z; c.nui1 = (T)p.la_nui1; c.nui2 = (T)p.la_nui2;
     c.beta = (T)p.lbm_beta; c.phi_inlet = (T)p.phi_inlet; c.sa_inject = (T)p.sa_inject; c.cos_theta = (T)p.cos_theta;
     c.rho_in = (T)p.rho_in; c.rho_out = (T)p.rho_out; c.relaxation = (T)p.relaxation;
+    c.rho_ref = (T)h->rho_ref;
     return c;
 }
 
@@ -492,7 +494,8 @@ int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0) {
     return 0;
 }
 
-int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w) {
+static int init_populations(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w,
+                            bool relaxed) {
     if (!h) return -1;
     const int64_t n = h->d.S1;
     const double* src[4] = {rho, u, v, w};
@@ -502,8 +505,20 @@ int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, 
             HIPCHK(h, hipMalloc(&dv[a], n * sizeof(double)));
             HIPCHK(h, hipMemcpy(dv[a], src[a], n * sizeof(double), hipMemcpyHostToDevice));
         }
-    if (h->prec == MFLBM_FP64) Launch<double>::init_eq(fields<double>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
-    else Launch<float>::init_eq(fields<float>(h), dv[0], dv[1], dv[2], dv[3], h->stream);
+    if (h->prec == MFLBM_FP64) {
+        Launch<double>::init_eq(fields<double>(h), dv[0], dv[1], dv[2], dv[3], relaxed, 1.0, h->stream);
+    } else {
+        // the FP32 deviations are taken from the mean initial density
+        double mean = 1.0;
+        if (rho) {
+            mean = 0.0;
+            for (int64_t c = 0; c < n; ++c) mean += rho[c];
+            mean /= (double)n;
+        }
+        if (mean != h->rho_ref) invalidate_graphs(h);
+        h->rho_ref = mean;
+        Launch<float>::init_eq(fields<float>(h), dv[0], dv[1], dv[2], dv[3], relaxed, mean, h->stream);
+    }
     cells_transfer(h, false);
     hipError_t e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(h->stream);
@@ -511,6 +526,14 @@ int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, 
         if (dv[a]) (void)hipFree(dv[a]);
     if (e != hipSuccess || e2 != hipSuccess) return fail(h, "init_equilibrium failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     return 0;
+}
+
+int mflbm_init_equilibrium(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w) {
+    return init_populations(h, rho, u, v, w, false);
+}
+
+int mflbm_init_relaxed(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w) {
+    return init_populations(h, rho, u, v, w, true);
 }
 
 int mflbm_color_gradient(mflbm_handle* h) {
@@ -566,6 +589,28 @@ int mflbm_profile(mflbm_handle* h, int32_t ntime0, int32_t nsteps, mflbm_kernel_
     return 0;
 }
 
+// FP32 handles store colour 1 and the deviation of the total from the rest state at
+// rho_ref (ShiftedStorage); the PDF field crosses the ABI in the reference's two-colour layout.
+static void shifted_to_colours(float* a, int64_t s1, double rho_ref) {
+    for (int s = 0; s < 19; ++s) {
+        const double r = rho_ref * rest_weight(s);
+        float* g1 = a + (int64_t)s * s1;
+        float* x = a + (int64_t)(s + 19) * s1;
+        for (int64_t n = 0; n < s1; ++n) x[n] = (float)(((double)x[n] + r) - (double)g1[n]);
+    }
+}
+
+static void colours_to_shifted(const float* src, float* a, int64_t s1, double rho_ref) {
+    for (int s = 0; s < 19; ++s) {
+        const double r = rho_ref * rest_weight(s);
+        const float* g1 = src + (int64_t)s * s1;
+        const float* g2 = src + (int64_t)(s + 19) * s1;
+        float* x = a + (int64_t)(s + 19) * s1;
+        std::memcpy(a + (int64_t)s * s1, g1, s1 * sizeof(float));
+        for (int64_t n = 0; n < s1; ++n) x[n] = (float)(((double)g1[n] + (double)g2[n]) - r);
+    }
+}
+
 int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes) {
     if (!h || !dst) return -1;
     int64_t n; size_t es; void* ptr;
@@ -580,6 +625,7 @@ int mflbm_download(mflbm_handle* h, int32_t field, void* dst, size_t bytes) {
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(dst, ptr, bytes, hipMemcpyDeviceToHost));
+    if (field == MFLBM_PDF && h->prec == MFLBM_FP32) shifted_to_colours((float*)dst, h->d.S1, h->rho_ref);
     return 0;
 }
 
@@ -591,7 +637,13 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
     if (bytes != (size_t)n * es) return fail(h, "field %d: expected %zu bytes, got %zu", field, (size_t)n * es, bytes);
     if (materialize_gradient(h)) return -1;  // the fields not uploaded keep the step state
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
+    if (field == MFLBM_PDF && h->prec == MFLBM_FP32) {
+        std::vector<float> tmp((size_t)n);
+        colours_to_shifted((const float*)src, tmp.data(), h->d.S1, h->rho_ref);
+        HIPCHK(h, hipMemcpy(ptr, tmp.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
+    }
     if (field == MFLBM_PDF) cells_transfer(h, false);
     if (field == MFLBM_CN_X || field == MFLBM_CN_Y || field == MFLBM_CN_Z || field == MFLBM_C_NORM || field == MFLBM_CURV)
         aux_refresh(h);

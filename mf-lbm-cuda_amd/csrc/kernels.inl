@@ -40,6 +40,11 @@ struct W {
     static constexpr T w0 = T(1.0) / T(3.0);
     static constexpr T w1 = T(1.0) / T(18.0);
     static constexpr T w2 = T(1.0) / T(36.0);
+    // rest state of the collision below: e = -11 rho and e2 = e2c1 rho = 0 give
+    // f = rho (4/21, 13/126, 1/63) rather than the D3Q19 weights (whose e2 is 3 rho)
+    static constexpr T r0 = T(4.0) / T(21.0);
+    static constexpr T r1 = T(13.0) / T(126.0);
+    static constexpr T r2 = T(1.0) / T(63.0);
 };
 
 template <typename T>
@@ -48,16 +53,37 @@ __device__ __forceinline__ T wq(int q) { return q == 0 ? W<T>::w0 : (q <= 6 ? W<
 // ---------------------------------------------------------------------------------------
 // Collision + LKR recolouring of one node (src/main_iteration_GPU.cu:116-345).
 // ---------------------------------------------------------------------------------------
+// With ShiftedStorage (FP32) g2 / o2 hold the deviation of the total population from the
+// rest state at rho_ref; the moments are then those of the deviation, in which only
+// m_rho (rho_ref) and m_e (-11 rho_ref) of the rest state differ from zero, so the same
+// relaxation applies with the density deviation in place of the density.
 template <typename T>
 __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
                                           T cnz, T ftmp, T (&o1)[19], T (&o2)[19]) {
+    constexpr bool SH = ShiftedStorage<T>::value;
     T ft[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) ft[q] = g1[q] + g2[q];
+    for (int q = 0; q < 19; ++q) ft[q] = SH ? g2[q] : g1[q] + g2[q];
     T rho1 = g1[0], rho2 = g2[0];
 #pragma unroll
     for (int q = 1; q < 19; ++q) { rho1 = rho1 + g1[q]; rho2 = rho2 + g2[q]; }
-    const T phi = (rho1 - rho2) / (rho1 + rho2);
+    T den, m_rho;
+    if (SH) {
+        m_rho = rho2;  // sum of the deviations
+        den = P.rho_ref + m_rho;
+        // Colour 1 is stored whole, so where fluid 2 is absent its sum carries the float
+        // rounding of O(0.1) populations (~1e-8) against the exact total; left alone, that
+        // residue would read as a trace of fluid 2 and feed the colour gradient.  A
+        // fraction below 1e-6 is taken as none, which keeps such regions at phi = 1
+        // exactly, as colour 2 does on its own where fluid 1 is absent (there colour 1
+        // is exactly zero).
+        if (den - rho1 < T(1e-6) * den) rho1 = den;
+        rho2 = den - rho1;
+    } else {
+        den = rho1 + rho2;
+        m_rho = den;
+    }
+    const T phi = (rho1 - rho2) / (SH ? den : rho1 + rho2);
 
     // ftmp = 0.5 * gamma * curv * c_norm, evaluated in that order by k_curvature
     T tmp = ftmp;
@@ -66,7 +92,6 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
     const T s_nu = omega, s_e = T(1.19), s_e2 = T(1.4), s_pi = T(1.4), s_q = T(1.2), s_t = T(1.98);
 
-    const T den = rho1 + rho2;
     const T ux = ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] + T(0.5) * fx;
     const T uy = ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] + T(0.5) * fy;
     const T uz = ft[5] - ft[6] + ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18] + T(0.5) * fz;
@@ -82,7 +107,6 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     T sum8 = ft[3] + ft[4] - ft[5] - ft[6];
     T sum9 = ft[7] + ft[8] + ft[9] + ft[10] - ft[11] - ft[12] - ft[13] - ft[14];
 
-    T m_rho = den;
     T m_e = T(-30.0) * ft[0] - T(11.0) * sum1 + T(8.0) * sum2;
     T m_e2 = T(12.0) * ft[0] - T(4.0) * sum1 + sum2;
     T m_jx = ft[1] - ft[2] + sum3;
@@ -104,8 +128,8 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 
     const T c23 = T(-0.666666666666666667), c13 = T(0.333333333333333333);
     const T e2c1 = T(0.0), e2c2 = T(-475.0) / T(63.0), oxx = T(0.0);
-    m_e = m_e - s_e * (m_e - (T(-11.0) * den + T(19.0) * u2)) + (T(38.0) - T(19.0) * s_e) * (fx * ux + fy * uy + fz * uz);
-    m_e2 = m_e2 - s_e2 * (m_e2 - (e2c1 * den + e2c2 * u2)) + (T(-11.0) + T(5.5) * s_e2) * (fx * ux + fy * uy + fz * uz);
+    m_e = m_e - s_e * (m_e - (T(-11.0) * m_rho + T(19.0) * u2)) + (T(38.0) - T(19.0) * s_e) * (fx * ux + fy * uy + fz * uz);
+    m_e2 = m_e2 - s_e2 * (m_e2 - (e2c1 * m_rho + e2c2 * u2)) + (T(-11.0) + T(5.5) * s_e2) * (fx * ux + fy * uy + fz * uz);
     m_jx = m_jx + fx;
     m_qx = m_qx - s_q * (m_qx - (c23 * ux)) + (c23 + c13 * s_q) * fx;
     m_jy = m_jy + fy;
@@ -167,8 +191,8 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     f[18] = sum4 - sum6 - sum7 - sum8 * T(2.0) + m_pyz - m_ty + m_tz;
 
     const T tmp1 = rho1 / den;
-    o1[0] = tmp1 * f[0];
-    o2[0] = f[0] * (T(1.0) - tmp1);
+    o1[0] = tmp1 * (SH ? f[0] + P.rho_ref * T(rest_weight(0)) : f[0]);
+    o2[0] = SH ? f[0] : f[0] * (T(1.0) - tmp1);
     tmp = rho1 * rho2 * P.beta / den;
     const T rk2 = T(1.0) / sqrt(T(2.0)) / T(36.0);
     const T w1 = T(1.0) / T(18.0);
@@ -179,8 +203,8 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const T w = q <= 6 ? w1 : rk2;
-        o1[q] = tmp1 * f[q] + w * tmp * (pr[q]);
-        o2[q] = f[q] - o1[q];
+        o1[q] = tmp1 * (SH ? f[q] + P.rho_ref * T(rest_weight(q)) : f[q]) + w * tmp * (pr[q]);
+        o2[q] = SH ? f[q] : f[q] - o1[q];
     }
     return phi;
 }
@@ -287,6 +311,28 @@ __global__ __launch_bounds__(256) void k_periodic_phi(T* __restrict__ phi, Dims 
     }
 }
 
+// One colour population of a cell in shifted storage (colour 1 and the total's deviation
+// from the rest state), read and written as the plain colour value; writing colour 1 keeps
+// colour 2 unchanged.
+template <typename T>
+struct ColourRef {
+    T* g1;
+    T* dev;
+    T rest;
+    bool colour2;
+    __device__ operator T() const { return colour2 ? (*dev + rest) - *g1 : *g1; }
+    __device__ ColourRef& operator=(T v) {
+        if (colour2) {
+            *dev = (*g1 + v) - rest;
+        } else {
+            const T g2 = (*dev + rest) - *g1;
+            *g1 = v;
+            *dev = (v + g2) - rest;
+        }
+        return *this;
+    }
+};
+
 // Zou-He inlet/outlet, one thread per (i, j).  zmap[plane][slot][b * (nx+2) + a] gives the
 // cell of (a, b, k) for the six planes k = 0, 1, 2, nz-1, nz, nz+1.
 template <typename T, bool AFTER_ODD>
@@ -298,14 +344,23 @@ __global__ __launch_bounds__(256) void k_zou_he(Sparse<T> S, Fields<T> F, StepCo
     const int i = t % d.nx + 1, j = t / d.nx + 1, nz = d.nz;
     const int64_t NS = S.ns;
     const int64_t plane = (int64_t)19 * d.X1 * d.Y1;
-    auto f = [&](int a, int b, int c, int e, int g) -> T& {
+    auto cell_of = [&](int a, int b, int c, int e) -> int64_t {
         const int pl = c <= 2 ? c : 3 + (c - (nz - 1));
-        const int cell = zmap[pl * plane + (int64_t)e * d.X1 * d.Y1 + b * d.X1 + a];
-        return S.f[(e + 19 * g) * NS + cell];
+        return zmap[pl * plane + (int64_t)e * d.X1 * d.Y1 + b * d.X1 + a];
     };
     auto wall = [&](int a, int b, int c) -> int { return F.walls[i2(d, a, b, c)]; };
     auto ph = [&](int a, int b, int c) -> T& { return F.phi[i4(d, a, b, c)]; };
-    zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
+    if constexpr (ShiftedStorage<T>::value) {
+        auto f = [&](int a, int b, int c, int e, int g) -> ColourRef<T> {
+            const int64_t cell = cell_of(a, b, c, e);
+            return ColourRef<T>{S.f + e * NS + cell, S.f + (e + 19) * NS + cell, P.rho_ref * (T)rest_weight(e),
+                                g == 1};
+        };
+        zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
+    } else {
+        auto f = [&](int a, int b, int c, int e, int g) -> T& { return S.f[(e + 19 * g) * NS + cell_of(a, b, c, e)]; };
+        zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
+    }
 }
 
 // Dense reference layout <-> compact storage.  Fluid cells: t < nf; extra cells: the
@@ -579,13 +634,30 @@ __global__ __launch_bounds__(256) void k_init_phi(Fields<T> F, int option, doubl
     F.phi[t] = (T)v;
 }
 
-template <typename T>
+template <typename T, bool RELAXED>
 __global__ __launch_bounds__(256) void k_init_eq(Fields<T> F, const double* rho, const double* u, const double* v,
-                                                 const double* w) {
+                                                 const double* w, double rho_ref) {
     const Dims& d = F.d;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= d.S1) return;
     const int i = t % d.X1, j = (t / d.X1) % d.Y1, k = t / (d.X1 * d.Y1);
+    if (ShiftedStorage<T>::value) {
+        // colour 1 and the total's deviation from the rest state at rho_ref, in FP64
+        const double uu = u ? u[t] : 0.0, vv = v ? v[t] : 0.0, ww = w ? w[t] : 0.0, rr = rho ? rho[t] : 1.0;
+        const double usq = uu * uu + vv * vv + ww * ww;
+        const double r1 = rr * (1.0 + (double)F.phi[i4(d, i, j, k)]) * 0.5;
+        const double cu[19] = {0.0, uu, -uu, vv, -vv, ww, -ww, uu + vv, -uu + vv, uu - vv, -uu - vv,
+                               uu + ww, -uu + ww, uu - ww, -uu - ww, vv + ww, -vv + ww, vv - ww, -vv - ww};
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const double wqq = q == 0 ? 1.0 / 3.0 : (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+            const double rqq = RELAXED ? rest_weight(q) : wqq;
+            const double shape = wqq * (3.0 * cu[q] + 4.5 * cu[q] * cu[q] - 1.5 * usq);
+            F.pdf[t + d.S1 * q] = (T)(r1 * rqq + r1 * shape);
+            F.pdf[t + d.S1 * (q + 19)] = (T)((rr - rho_ref) * rqq + rho_ref * (rqq - rest_weight(q)) + rr * shape);
+        }
+        return;
+    }
     const T uu = u ? (T)u[t] : T(0), vv = v ? (T)v[t] : T(0), ww = w ? (T)w[t] : T(0), rr = rho ? (T)rho[t] : T(1);
     const T usq = uu * uu + vv * vv + ww * ww;
     const T ph = F.phi[i4(d, i, j, k)];
@@ -595,11 +667,12 @@ __global__ __launch_bounds__(256) void k_init_eq(Fields<T> F, const double* rho,
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         const T r = rc[g];
-        F.pdf[t + d.S1 * (19 * g)] = r * W<T>::w0 + r * W<T>::w0 * (T(-1.5) * usq);
+        F.pdf[t + d.S1 * (19 * g)] = r * (RELAXED ? W<T>::r0 : W<T>::w0) + r * W<T>::w0 * (T(-1.5) * usq);
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
             const T wqq = q <= 6 ? W<T>::w1 : W<T>::w2;
-            F.pdf[t + d.S1 * (q + 19 * g)] = r * wqq + r * wqq * (T(3.0) * cu[q] + T(4.5) * cu[q] * cu[q] - T(1.5) * usq);
+            const T rqq = RELAXED ? (q <= 6 ? W<T>::r1 : W<T>::r2) : wqq;
+            F.pdf[t + d.S1 * (q + 19 * g)] = r * rqq + r * wqq * (T(3.0) * cu[q] + T(4.5) * cu[q] * cu[q] - T(1.5) * usq);
         }
     }
 }
@@ -612,16 +685,17 @@ __global__ __launch_bounds__(256) void k_init_eq(Fields<T> F, const double* rho,
 template <typename T>
 __device__ __forceinline__ void macro_node(const Sparse<T>& S, const StepConst<T>& P, int t, double& rho, double& u,
                                            double& v, double& w) {
+    constexpr bool SH = ShiftedStorage<T>::value;
     T ft[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) ft[q] = S.f[q * S.ns + t] + S.f[(q + 19) * S.ns + t];
+    for (int q = 0; q < 19; ++q) ft[q] = SH ? S.f[(q + 19) * S.ns + t] : S.f[q * S.ns + t] + S.f[(q + 19) * S.ns + t];
     T r = ft[0];
 #pragma unroll
     for (int q = 1; q < 19; ++q) r = r + ft[q];
     const T* a = S.aux + 4 * (int64_t)t;
     const T tmp = a[3];
     const T fx = tmp * a[0], fy = tmp * a[1], fz = tmp * a[2] + P.force_z;
-    rho = (double)r;
+    rho = SH ? (double)P.rho_ref + (double)r : (double)r;
     u = (double)(ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] - T(0.5) * fx);
     v = (double)(ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] - T(0.5) * fy);
     w = (double)(ft[5] - ft[6] + ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18] - T(0.5) * fz);

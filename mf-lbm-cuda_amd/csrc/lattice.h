@@ -72,6 +72,7 @@ struct Fields {
 template <typename T>
 struct StepConst {
     T gamma, force_z, nui1, nui2, beta, phi_inlet, sa_inject, cos_theta, rho_in, rho_out, relaxation;
+    T rho_ref;  // ShiftedStorage: density of the rest state the deviations are taken from
 };
 
 struct NodeLists {
@@ -83,10 +84,28 @@ struct NodeLists {
     int n_solid_bnd;
 };
 
+// Rest state of the collision at rho = 1: (4/21, 13/126 x 6, 1/63 x 12) (MRT e = -11 rho,
+// fourth-order moment 0).
+MFLBM_HD double rest_weight(int q) { return q == 0 ? 4.0 / 21.0 : (q <= 6 ? 13.0 / 126.0 : 1.0 / 63.0); }
+
+// Population storage of a precision.  FP64 keeps the reference's two colour populations
+// (slot s: colour 1, slot s + 19: colour 2).  FP32 keeps colour 1 in slot s and, in slot
+// s + 19, the total f_s - rho_ref * rest_weight(s): the deviation of the total population
+// from the collision's rest state at the reference density rho_ref (StepConst; the mean
+// density of the initial state).  Populations are O(0.1) while the flow lives in deviations of
+// O(1e-6 .. 1e-3); a float holding the whole population resolves 7.5e-9 of it, a float
+// holding the deviation resolves it to 6e-8 relative.  Density, momentum and the
+// collision's moments are computed from the deviations, in which the rest state cancels
+// exactly; colour 2 is f - colour 1 where it is needed.
+template <typename T>
+struct ShiftedStorage { static constexpr bool value = false; };
+template <>
+struct ShiftedStorage<float> { static constexpr bool value = true; };
+
 // Fluid-compact population storage (see kernels.inl for the cell layout).
 template <typename T>
 struct Sparse {
-    T* f;                           // 38 * ns, slot (s + 19 g) at offset (s + 19 g) * ns
+    T* f;                           // 38 * ns, slot (s + 19 g) at offset (s + 19 g) * ns (ShiftedStorage)
     int64_t ns;                     // cells per slot array
     int nf;                         // interior fluid nodes
     const int32_t* __restrict__ nb; // 18 * nf

@@ -23,7 +23,7 @@ struct Launch {
                       int n_ext, bool to_dense, hipStream_t s);
     static void init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, hipStream_t s);
     static void init_eq(const Fields<T>& F, const double* rho, const double* u, const double* v, const double* w,
-                        hipStream_t s);
+                        bool relaxed, double rho_ref, hipStream_t s);
     static void macro(const Sparse<T>& S, const StepConst<T>& P, const Dims& d, double* rho, double* u, double* v,
                       double* w, hipStream_t s);
     static void monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi, const Dims& d, const int32_t* zstart,

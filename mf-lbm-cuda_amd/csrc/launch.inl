@@ -105,8 +105,9 @@ void Launch<T>::init_phi(const Fields<T>& F, int option, double z0, double phi_i
 
 template <typename T>
 void Launch<T>::init_eq(const Fields<T>& F, const double* rho, const double* u, const double* v, const double* w,
-                        hipStream_t s) {
-    dev::k_init_eq<T><<<blocks(F.d.S1), 256, 0, s>>>(F, rho, u, v, w);
+                        bool relaxed, double rho_ref, hipStream_t s) {
+    if (relaxed) dev::k_init_eq<T, true><<<blocks(F.d.S1), 256, 0, s>>>(F, rho, u, v, w, rho_ref);
+    else dev::k_init_eq<T, false><<<blocks(F.d.S1), 256, 0, s>>>(F, rho, u, v, w, rho_ref);
 }
 
 template <typename T>

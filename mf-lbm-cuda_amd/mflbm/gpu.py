@@ -24,6 +24,7 @@ HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv"
 
 # Every symbol declared in include/mflbm.h.
 EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_equilibrium",
+           "mflbm_init_relaxed",
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
            "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe"]
 
@@ -76,6 +77,7 @@ def load():
     L.mflbm_last_error.restype = C.c_char_p
     L.mflbm_init_phi.argtypes = [vp, C.c_int32, C.c_double]
     L.mflbm_init_equilibrium.argtypes = [vp, dp, dp, dp, dp]
+    L.mflbm_init_relaxed.argtypes = [vp, dp, dp, dp, dp]
     L.mflbm_color_gradient.argtypes = [vp]
     L.mflbm_step.argtypes = [vp, C.c_int32]
     L.mflbm_run.argtypes = [vp, C.c_int32, C.c_int32]
@@ -91,8 +93,9 @@ def load():
     L.mflbm_field_size.restype = C.c_int64
     L.mflbm_bandwidth_probe.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.mflbm_bandwidth_probe.restype = C.c_int
-    for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_color_gradient", "mflbm_step", "mflbm_run",
-                 "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro", "mflbm_monitor", "mflbm_sync"):
+    for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_init_relaxed", "mflbm_color_gradient",
+                 "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro",
+                 "mflbm_monitor", "mflbm_sync"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -160,9 +163,11 @@ class Solver:
     def init_phi(self, option, interface_z0):
         self._check(load().mflbm_init_phi(self.h, int(option), float(interface_z0)))
 
-    def init_equilibrium(self, rho=None, u=None, v=None, w=None):
+    def init_equilibrium(self, rho=None, u=None, v=None, w=None, relaxed=False):
+        """relaxed: rest weights of the collision (mflbm_init_relaxed) instead of the D3Q19 ones"""
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w)]
-        self._check(load().mflbm_init_equilibrium(self.h, *[_dp(a) for a in arrs]))
+        fn = load().mflbm_init_relaxed if relaxed else load().mflbm_init_equilibrium
+        self._check(fn(self.h, *[_dp(a) for a in arrs]))
 
     def color_gradient(self):
         self._check(load().mflbm_color_gradient(self.h))

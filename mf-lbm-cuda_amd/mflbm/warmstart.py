@@ -1,0 +1,300 @@
+"""Learned warm start of the flow solve.
+
+The reference starts every run from rest (u = 0, src/Init_multiphase.cpp:569-572) and
+builds the populations from (rho, u, v, w, phi) with the D3Q19 weights 1/3, 1/18, 1/36
+(src/Init_multiphase.cpp:389-443).  Here the start has two parts:
+
+1. Relaxed initialisation (mflbm_init_relaxed).  The MRT collision relaxes the fourth-order
+   moment e2 to 0, so its rest state is rho (4/21, 13/126, 1/63), not the D3Q19 weights
+   (e2 = 3 rho).  At fluid nodes the first collision removes the difference locally; in the
+   bounce-back links it returns to the fluid as a mass pulse of about -0.048 (axis links)
+   and +0.012 (diagonal links) per wall link, which then has to spread through the pore
+   space by pressure diffusion -- the slowest process of a single-phase start.  Starting
+   at the collision's own rest weights removes the pulse.  The mean density is set from
+   the cold start's total mass (relaxed_mean_density): the same amount of fluid, so the
+   same steady state.
+2. Learned fields.  A 3-D convolutional U-Net (mflbm.nn, trained on the device by the
+   project's own kernels) maps the pore geometry -- solid mask and signed distance to the
+   solid -- to the steady velocity and pressure (density) fields, which enter through the
+   same initialisation.  The predicted density perturbation is shifted to zero mean over the
+   fluid, so the mass stays that of the cold start.  With Guo forcing the populations of the
+   first step are pre-collision values whose momentum is rho u - F / 2, so F / 2 is taken
+   off the predicted velocity.
+
+The solver stays the judge: every start runs to the same steady criterion.  `evaluate`
+reports, per held-out geometry, the reference's cold start (FP64), the relaxed start from
+rest (the method without the network) and the learned warm start, the last two in the
+chosen precision (FP32 by default).
+
+Data: single-phase body-force-driven flow (phi = 1 everywhere, body force 1e-6 along the flow
+axis, periodic along flow and one transverse axis, walls on the other) through seeded sphere
+packs, porosity drawn from [0.30, 0.50] and radius from [4, 10]; each sample is run to its
+steady state by the solver itself.  Targets are the velocities scaled by nu / F (a length
+squared, independent of the driving force) and the density perturbation scaled by 1 / (3 F)
+(a length), each normalised by one constant of the training set.
+
+Steady criterion: the relative L2 change of the velocity field over `check` steps falls
+below `tol` (default 1e-5 per 100 steps, about 1e-7 per step).
+
+    python -m mflbm.warmstart --train 48 --eval 8 --epochs 30
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import cases, config, geometry, gpu, nn
+
+BODY_FORCE = 1e-6
+
+
+def single_phase_case(seed, n=64):
+    rng = np.random.default_rng(10_000 + seed)
+    porosity = float(rng.uniform(0.30, 0.50))
+    radius = float(rng.uniform(4.0, 10.0))
+    geo = geometry.sphere_pack((n, n, n), radius, porosity, seed, periodic=(True, True, False))
+    ctrl = cases.periodic_control(n=(n, n, n), body_force=BODY_FORCE, option=1, interface_z0=float(n + 10))
+    return geo, config.derive(ctrl), {"seed": seed, "porosity": porosity, "radius": radius}
+
+
+def solid_with_walls(geo):
+    """the solver's solid nodes: the geometry plus the x-wall planes it adds"""
+    s = geo.astype(bool).copy()
+    s[:, :, 0] = True
+    s[:, :, -1] = True
+    return s
+
+
+def signed_distance(solid):
+    """distance to the nearest solid node in the pore (> 0), minus the distance to the
+    nearest pore node in the solid; periodic images along z and y are included."""
+    from scipy import ndimage
+    pad = ((16, 16), (16, 16), (0, 0))
+    s = np.pad(solid.astype(bool), pad, mode="wrap")
+    d_out = ndimage.distance_transform_edt(~s)
+    d_in = ndimage.distance_transform_edt(s)
+    sd = (d_out - d_in)[16:-16, 16:-16, :]
+    return sd
+
+
+def features(solid, channels=8):
+    """network input: channel 0 = solid mask, channel 1 = signed distance / 8 (clipped)"""
+    x = np.zeros(solid.shape + (channels,), np.float32)
+    x[..., 0] = solid
+    x[..., 1] = np.clip(signed_distance(solid) / 8.0, -1.0, 1.0)
+    return x
+
+
+D3Q19 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, 1, 0), (1, -1, 0),
+         (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1)]
+
+
+def relaxed_mean_density(geo):
+    """Mean fluid density of the steady state the reference's cold start reaches, for a
+    start at the collision's rest weights.  Mass is conserved: the cold start holds
+    N_fluid * 1 in the fluid plus the D3Q19 weight of each bounce-back link (the population
+    a solid neighbour returns on the second step); at rest the same links hold rho times
+    the relaxed weights.  So rho = (N + sum_links w) / (N + sum_links r)."""
+    solid = solid_with_walls(geo)
+    fluid = ~solid
+    nf = int(fluid.sum())
+    w_std = r_rel = 0.0
+    for e in D3Q19:
+        # fluid x whose neighbour x - e is solid: one live bounce-back slot
+        links = int((fluid & np.roll(solid, shift=(e[2], e[1], e[0]), axis=(0, 1, 2))).sum())
+        axis = sum(abs(c) for c in e) == 1
+        w_std += links * (1.0 / 18.0 if axis else 1.0 / 36.0)
+        r_rel += links * (13.0 / 126.0 if axis else 1.0 / 63.0)
+    return (nf + w_std) / (nf + r_rel)
+
+
+def run_to_steady(sim, params, init=None, relaxed=False, tol=1e-5, check=100, max_steps=60000):
+    """Initialise from init = (rho, u, v, w) on the s1 grid (None = the reference's rest
+    state) with the D3Q19 or, relaxed, the collision's rest weights, and step until the
+    relative L2 change of the velocity over `check` steps is below tol.
+    Returns (steps, velocity (3, nz, ny, nx), density (nz, ny, nx))."""
+    sim.init_phi(params["initial_fluid_distribution_option"], params["interface_z0"])
+    sim.init_equilibrium(*(init or (None, None, None, None)), relaxed=relaxed)
+    sim.color_gradient()
+    prev = None
+    t = 1
+    while t <= max_steps:
+        sim.run(t, check)
+        t += check
+        rho, u, v, w = sim.macro()
+        vel = np.stack([u[1:-1, 1:-1, 1:-1], v[1:-1, 1:-1, 1:-1], w[1:-1, 1:-1, 1:-1]])
+        if prev is not None:
+            change = np.linalg.norm(vel - prev) / max(np.linalg.norm(vel), 1e-300)
+            if change < tol:
+                break
+        prev = vel
+    return t - 1, vel, rho[1:-1, 1:-1, 1:-1]
+
+
+def permeability(vel, params):
+    """k = nu <w> / F (Darcy velocity over the whole cross-section)"""
+    nu = 1.0 / params["la_nui1"]
+    return float(vel[2].mean() * nu / BODY_FORCE)
+
+
+def to_s1(field):
+    """interior (nz, ny, nx) -> s1 grid with a periodic one-node halo"""
+    return np.pad(np.asarray(field, np.float64), 1, mode="wrap")
+
+
+def rest_state(geo):
+    """(rho, u, v, w) of the relaxed start from rest"""
+    rho = np.full(tuple(s + 2 for s in geo.shape), relaxed_mean_density(geo))
+    return rho, None, None, None
+
+
+def generate(seeds, n=64, log=print):
+    """[(geometry, target (4, nz, ny, nx), params, meta)] with target = velocity * nu / F and
+    the zero-mean density perturbation / (3 F) over the fluid"""
+    data = []
+    for s in seeds:
+        geo, params, meta = single_phase_case(s, n)
+        t0 = time.perf_counter()
+        sim = gpu.Solver(params, geo, gpu.FP64)
+        steps, vel, rho = run_to_steady(sim, params, rest_state(geo), relaxed=True)
+        sim.close()
+        meta.update(steps=steps, seconds=time.perf_counter() - t0, k=permeability(vel, params))
+        log(f"sample {s}: porosity {meta['porosity']:.3f} radius {meta['radius']:.1f} -> steady after {steps} "
+            f"steps, k = {meta['k']:.4f}")
+        nu = 1.0 / params["la_nui1"]
+        fluid = ~solid_with_walls(geo)
+        drho = np.where(fluid, rho - rho[fluid].mean(), 0.0)
+        target = np.concatenate([vel * nu / BODY_FORCE, (drho / (3.0 * BODY_FORCE))[None]]).astype(np.float32)
+        data.append((geo, target, params, meta))
+    return data
+
+
+class WarmStart:
+    def __init__(self, widths=(16, 32, 64, 128), seed=0):
+        self.net = nn.UNet(widths=widths, cin=8, cout=8, n_out=4, seed=seed)
+        self.scale = np.ones(4)
+
+    def _tensors(self, geo, target=None):
+        x = nn.DevArray.from_numpy(features(geo), "bf16")
+        if target is None:
+            return x, None, None
+        t = np.zeros(geo.shape + (8,), np.float32)
+        t[..., :4] = np.moveaxis(target, 0, -1) / self.scale
+        mask = (~solid_with_walls(geo)).astype(np.float32)
+        return x, nn.DevArray.from_numpy(t, "f32"), nn.DevArray.from_numpy(mask, "f32")
+
+    def train(self, data, epochs=30, lr=1e-3, log=print):
+        fl = [~solid_with_walls(g) for g, _, _, _ in data]
+        self.scale = np.array([np.mean([np.abs(t[c][f]).mean() for (_, t, _, _), f in zip(data, fl)])
+                               for c in range(3)] + [np.mean([t[3][f].std() for (_, t, _, _), f in zip(data, fl)])])
+        self.scale[:3] = self.scale[2]  # one velocity scale, set by the flow-direction mean
+        self.scale = np.maximum(self.scale, 1e-12)
+        tensors = [self._tensors(g, t) for g, t, _, _ in data]
+        hist = []
+        rng = np.random.default_rng(0)
+        for ep in range(epochs):
+            tot = 0.0
+            for i in rng.permutation(len(tensors)):
+                x, t, m = tensors[i]
+                a = self.net.forward(x)
+                part, inv, gy = self.net.loss_grad(a["y"], t, m)
+                self.net.backward(a, gy)
+                self.net.adam(lr)
+                tot += float(part.numpy().sum()) * inv
+            hist.append(tot / len(tensors))
+            log(f"epoch {ep + 1}: loss {hist[-1]:.4e}")
+        return hist
+
+    def predict(self, geo, params):
+        """(rho, u, v, w) on the s1 grid for mflbm_init_relaxed"""
+        x, _, _ = self._tensors(geo)
+        y = self.net.forward(x)["y"].numpy()
+        nu = 1.0 / params["la_nui1"]
+        fluid = ~solid_with_walls(geo)
+        vel = [np.where(fluid, y[..., c] * self.scale[c] * BODY_FORCE / nu, 0.0) for c in range(3)]
+        vel[2] = vel[2] - 0.5 * BODY_FORCE * fluid  # pre-collision momentum of the first step
+        drho = y[..., 3] * self.scale[3] * 3.0 * BODY_FORCE
+        rbar = relaxed_mean_density(geo)
+        rho = np.where(fluid, rbar + drho - drho[fluid].mean(), rbar)
+        return tuple(to_s1(a) for a in (rho, *vel))
+
+
+def evaluate(ws, seeds, n=64, precision=gpu.FP32, log=print):
+    """Per held-out geometry: the reference's cold start in FP64 (the judge), the relaxed
+    start from rest and the learned warm start, both in `precision`."""
+    rows = []
+    for s in seeds:
+        geo, params, meta = single_phase_case(s, n)
+        row = dict(meta)
+        sim = gpu.Solver(params, geo, gpu.FP64)
+        t0 = time.perf_counter()
+        row["cold_steps"], cold, _ = run_to_steady(sim, params)
+        row["cold_s"] = time.perf_counter() - t0
+        sim.close()
+        k_cold = permeability(cold, params)
+        row["k_cold"] = k_cold
+        sim = gpu.Solver(params, geo, precision)
+        for name in ("relaxed", "warm"):
+            t0 = time.perf_counter()
+            init = ws.predict(geo, params) if name == "warm" else rest_state(geo)
+            steps, vel, _ = run_to_steady(sim, params, init, relaxed=True)
+            row[f"{name}_steps"] = steps
+            row[f"{name}_s"] = time.perf_counter() - t0
+            row[f"k_{name}"] = permeability(vel, params)
+            row[f"k_{name}_rel_diff"] = abs(row[f"k_{name}"] - k_cold) / abs(k_cold)
+        sim.close()
+        rows.append(row)
+        log(f"held-out {s}: cold {row['cold_steps']} steps {row['cold_s']:.2f} s, relaxed {row['relaxed_steps']} "
+            f"steps {row['relaxed_s']:.2f} s, warm {row['warm_steps']} steps {row['warm_s']:.2f} s; k {k_cold:.4f} / "
+            f"{row['k_relaxed']:.4f} / {row['k_warm']:.4f}")
+    return rows
+
+
+def summarise(rows):
+    tot = {k: float(sum(r[k] for r in rows)) for k in ("cold_steps", "relaxed_steps", "warm_steps", "cold_s",
+                                                        "relaxed_s", "warm_s")}
+    return {"mean_cold_steps": tot["cold_steps"] / len(rows), "mean_relaxed_steps": tot["relaxed_steps"] / len(rows),
+            "mean_warm_steps": tot["warm_steps"] / len(rows),
+            "warm_vs_cold_wall": tot["warm_s"] / tot["cold_s"], "relaxed_vs_cold_wall": tot["relaxed_s"] / tot["cold_s"],
+            "warm_vs_relaxed_steps": tot["warm_steps"] / tot["relaxed_steps"],
+            "max_k_warm_rel_diff": float(max(r["k_warm_rel_diff"] for r in rows)),
+            "max_k_relaxed_rel_diff": float(max(r["k_relaxed_rel_diff"] for r in rows))}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--train", type=int, default=48)
+    ap.add_argument("--eval", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32", help="of the relaxed and warm solves")
+    ap.add_argument("--out", default=None, help="write the JSON summary here as well")
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    data = generate(range(args.train), args.n, log)
+    t_data = time.perf_counter() - t0
+    ws = WarmStart()
+    log(f"network parameters: {ws.net.n_params()}")
+    t0 = time.perf_counter()
+    hist = ws.train(data, args.epochs, args.lr, log)
+    t_train = time.perf_counter() - t0
+    prec = gpu.FP32 if args.precision == "fp32" else gpu.FP64
+    rows = evaluate(ws, range(1000, 1000 + args.eval), args.n, prec, log)
+    res = {"n": args.n, "train_samples": args.train, "epochs": args.epochs, "params": ws.net.n_params(),
+           "precision": args.precision, "data_seconds": t_data, "train_seconds": t_train, "loss_first": hist[0],
+           "loss_last": hist[-1], **summarise(rows), "eval": rows}
+    text = json.dumps(res)
+    if args.out:
+        with open(args.out, "w") as fh:
+            fh.write(text + "\n")
+    print(text, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -77,6 +77,7 @@ const int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 1
 struct Consts {
     double w[19];
     double w1, w2, rk2;
+    double r0, r1, r2;  // rest weights of the collision's equilibrium
     double iso4[2];
     double iso8[7];
     double c19, c2394, c252, c72;   // MRT back-transform coefficients
@@ -88,6 +89,8 @@ struct Consts {
         for (int q = 7; q <= 18; ++q) w[q] = 1.0 / 36.0;
         w1 = 1.0 / 18.0;
         w2 = 1.0 / 36.0;
+        // rest state of the collision (e = -11 rho, e2 = e2c1 rho = 0): no reference counterpart
+        r0 = 4.0 / 21.0; r1 = 13.0 / 126.0; r2 = 1.0 / 63.0;
         rk2 = 1.0 / std::sqrt(2.0) / 36.0;           // includes/Fluid_multiphase.h:32
         iso4[0] = 1.0 / 6.0;  iso4[1] = 1.0 / 12.0;  // includes/Fluid_multiphase.h:34
         iso8[0] = 4.0 / 45.0; iso8[1] = 1.0 / 21.0; iso8[2] = 2.0 / 105.0; iso8[3] = 5.0 / 504.0;
@@ -351,8 +354,12 @@ void init_phi(OracleSim& S, int option, double interface_z0) {
     }
 }
 
-// equilibrium populations (src/Init_multiphase.cpp:389-443); rho/u/v/w on s1, null = rest
-void init_pdf(OracleSim& S, const double* rho, const double* u, const double* v, const double* w) {
+// equilibrium populations (src/Init_multiphase.cpp:389-443); rho/u/v/w on s1, null = rest.
+// relaxed: the constant terms r * w_q use the collision's own rest weights r0, r1, r2
+// (fourth-order moment e2 = 0 instead of the D3Q19 weights' 3 rho); the velocity terms
+// are unchanged.  The reference has no such option.
+void init_pdf(OracleSim& S, const double* rho, const double* u, const double* v, const double* w,
+              bool relaxed = false) {
     const int64_t nx = S.nx, ny = S.ny, nz = S.nz;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int64_t k = 0; k <= nz + 1; ++k)
@@ -365,7 +372,7 @@ void init_pdf(OracleSim& S, const double* rho, const double* u, const double* v,
                 double rc[2] = {rr * (1.0 + ph) * 0.5, rr * (1.0 - ph) * 0.5};
                 for (int g = 0; g < 2; ++g) {
                     double r = rc[g];
-                    S.pdf[n + S.S1 * (0 + 19 * g)] = r * K.w[0] + r * K.w[0] * (-1.5 * usqrt);
+                    S.pdf[n + S.S1 * (0 + 19 * g)] = r * (relaxed ? K.r0 : K.w[0]) + r * K.w[0] * (-1.5 * usqrt);
                     for (int q = 1; q < 19; ++q) {
                         // e.u keeps the reference's operand order: (+-a) (+-b) for diagonals
                         double cu;
@@ -378,7 +385,8 @@ void init_pdf(OracleSim& S, const double* rho, const double* u, const double* v,
                         if (b < 0) cu = ta;
                         else cu = e[b] > 0 ? ta + comp[b] : ta - comp[b];
                         double wq = (q <= 6) ? K.w1 : K.w2;
-                        S.pdf[n + S.S1 * (q + 19 * g)] = r * wq + r * wq * (3.0 * cu + 4.5 * cu * cu - 1.5 * usqrt);
+                        double rq = relaxed ? ((q <= 6) ? K.r1 : K.r2) : wq;
+                        S.pdf[n + S.S1 * (q + 19 * g)] = r * rq + r * wq * (3.0 * cu + 4.5 * cu * cu - 1.5 * usqrt);
                     }
                 }
             }
@@ -955,6 +963,10 @@ int oracle_init_phi(void* h, int option, double interface_z0) {
 
 void oracle_init_pdf(void* h, const double* rho, const double* u, const double* v, const double* w) {
     init_pdf(*static_cast<OracleSim*>(h), rho, u, v, w);
+}
+
+void oracle_init_pdf_relaxed(void* h, const double* rho, const double* u, const double* v, const double* w) {
+    init_pdf(*static_cast<OracleSim*>(h), rho, u, v, w, true);
 }
 
 void oracle_color_gradient(void* h) { gradient_chain(*static_cast<OracleSim*>(h)); }

@@ -54,6 +54,7 @@ def lib():
         L.oracle_init_phi.argtypes = [vp, C.c_int, C.c_double]
         L.oracle_init_phi.restype = C.c_int
         L.oracle_init_pdf.argtypes = [vp, dp, dp, dp, dp]
+        L.oracle_init_pdf_relaxed.argtypes = [vp, dp, dp, dp, dp]
         L.oracle_color_gradient.argtypes = [vp]
         L.oracle_step.argtypes = [vp, C.c_int]
         L.oracle_run.argtypes = [vp, C.c_int, C.c_int]
@@ -142,9 +143,11 @@ class Oracle:
         if lib().oracle_init_phi(self.h, int(option), float(interface_z0)) != 0:
             raise ValueError(f"unsupported initial_fluid_distribution_option {option}")
 
-    def init_pdf(self, rho=None, u=None, v=None, w=None):
+    def init_pdf(self, rho=None, u=None, v=None, w=None, relaxed=False):
+        """relaxed: rest weights 4/21, 13/126, 1/63 (the collision's own rest state)"""
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w)]
-        lib().oracle_init_pdf(self.h, *[_dp(a) for a in arrs])
+        fn = lib().oracle_init_pdf_relaxed if relaxed else lib().oracle_init_pdf
+        fn(self.h, *[_dp(a) for a in arrs])
 
     def color_gradient(self):
         lib().oracle_color_gradient(self.h)

@@ -121,6 +121,28 @@ def test_mixed_phase_field_bitwise():
     compare_state(o, g, tag="random phi step20")
 
 
+def test_relaxed_init_bitwise():
+    """mflbm_init_relaxed (the warm start's initialisation: rest weights 4/21, 13/126, 1/63)
+    from random density and velocity fields, then 20 steps, against the oracle."""
+    geo, p = CASES["drop_periodic_yz"]()
+    say("case=drop_periodic_yz relaxed init from random rho/u, 20 steps")
+    o, g = pair(geo, p)
+    o.init_phi(5, p["interface_z0"])
+    g.init_phi(5, p["interface_z0"])
+    rng = np.random.default_rng(21)
+    shp = o.shape(1)
+    rho = 1.0 + 1e-3 * rng.standard_normal(shp)
+    uvw = [1e-3 * rng.standard_normal(shp) for _ in range(3)]
+    o.init_pdf(rho, *uvw, relaxed=True)
+    g.init_equilibrium(rho, *uvw, relaxed=True)
+    o.color_gradient()
+    g.color_gradient()
+    compare_state(o, g, tag="relaxed init")
+    o.run(1, 20)
+    g.run(1, 20)
+    compare_state(o, g, tag="relaxed step20")
+
+
 def test_fused_gradient_short_z_runs(monkeypatch):
     """Workgroups marching 5-plane z-runs (many run boundaries) still match the oracle."""
     monkeypatch.setenv("MFLBM_GRAD_KCHUNK", "5")
@@ -209,3 +231,49 @@ def test_fp32_tracks_fp64():
     m_got = got["mass1"].sum() + got["mass2"].sum()
     assert abs(m_got - m_ref) / m_ref < 1e-4
     assert rel_linf(g.download("phi"), o.array("phi")) < 0.05
+
+
+def test_fp32_low_force_single_phase_velocity():
+    """FP32 storage keeps the deviation of the total population from the rest state, so
+    a flow of u ~ 1e-6 (body force 1e-6) is resolved, not lost under the float spacing of
+    O(0.1) populations.  From the relaxed start (no start-up pulse), 600 steps."""
+    from mflbm import warmstart
+    n = 24
+    geo = cases.geometry.sphere_pack((n, n, n), 3.0, 0.6, 4, periodic=(True, True, False))
+    p = config.derive(cases.periodic_control(n=(n, n, n), body_force=1e-6, option=1, interface_z0=float(n + 10)))
+    say("case=fp32 single phase, body force 1e-6, relaxed start, 600 steps")
+    o = O.Oracle(p, geo)
+    g = gpu.Solver(p, geo, gpu.FP32)
+    rho0 = np.full(o.shape(1), warmstart.relaxed_mean_density(geo))
+    o.init_phi(1, float(n + 10))
+    g.init_phi(1, float(n + 10))
+    o.init_pdf(rho0, relaxed=True)
+    g.init_equilibrium(rho0, relaxed=True)
+    o.color_gradient()
+    g.color_gradient()
+    o.run(1, 600)
+    g.run(1, 600)
+    rho_o, u_o, v_o, w_o = o.macro(zero_solid_phi=False)
+    rho_g, u_g, v_g, w_g = g.macro()
+    vo = np.stack([u_o, v_o, w_o])[:, 1:-1, 1:-1, 1:-1]
+    vg = np.stack([u_g, v_g, w_g])[:, 1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(vg - vo) / np.linalg.norm(vo)
+    fl = rho_g != 0
+    derr = np.abs(rho_g[fl] - rho_o[fl]).max()
+    say(f"fp32 velocity rel L2 {err:.2e}, density max abs {derr:.2e}")
+    assert err < 1e-4, err
+    assert derr < 1e-8, derr
+
+
+def test_fp32_pdf_round_trip():
+    """The FP32 population field crosses the ABI in the two-colour layout: download,
+    upload and download again returns the same values."""
+    geo, p = CASES["drainage_yper"]()
+    say("case=drainage_yper fp32 pdf download/upload")
+    g = gpu.Solver(p, geo, gpu.FP32)
+    g.setup()
+    g.run(1, 10)
+    a = g.download("pdf")
+    g.upload("pdf", a)
+    b = g.download("pdf")
+    assert np.abs(b.astype(np.float64) - a).max() <= 2e-8
