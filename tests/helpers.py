@@ -1,4 +1,6 @@
 """Shared helpers of the test suite: workload construction and array comparison."""
+import os
+
 import numpy as np
 
 from mflbm import cases, config
@@ -48,3 +50,19 @@ def assert_same(a, b, tol=0.0, what=""):
     else:
         r = rel_linf(a, b)
         assert r <= tol, f"{what}: relative Linf {r:.3e} > {tol:.1e}"
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mf-lbm-cuda_amd", "mflbm_run")
+
+
+def write_case(root, ctrl, geo, status="new_simulation"):
+    """Lay out a run directory the way the reference expects it: input/job_status.txt,
+    input/simulation_control.txt, input/Geometry_File_Path.txt and the .dat geometry."""
+    from mflbm import geometry
+    os.makedirs(os.path.join(root, "input", "geometry"), exist_ok=True)
+    config.write_control(os.path.join(root, "input", "simulation_control.txt"), ctrl)
+    geometry.write_dat(os.path.join(root, "input", "geometry", "sample1.dat"), geo)
+    with open(os.path.join(root, "input", "Geometry_File_Path.txt"), "w") as fh:
+        fh.write("geo_file_path_\tinput/geometry/sample1\n\ngeo_boundary_file_path_\tinput/geometry/sample1\n")
+    with open(os.path.join(root, "input", "job_status.txt"), "w") as fh:
+        fh.write(status)  # no newline: the reference compares the whole file
