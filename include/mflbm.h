@@ -147,6 +147,42 @@ int64_t mflbm_field_size(const mflbm_handle* h, int32_t field);
  * bandwidth figures and the profiler's FETCH_SIZE / WRITE_SIZE counters on the same GPU. */
 int mflbm_bandwidth_probe(mflbm_handle* h, int64_t bytes, int32_t repeats, double* ms_per_copy);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched single-phase ensemble.  No counterpart in the reference, which runs one domain
+ * per process: here n independent domains of nx*ny*nz -- walls on both x faces, periodic in
+ * y and z, body force along z, one fluid -- advance together, one launch per step over all
+ * of them, each until its velocity field is steady.  It produces the warm start's training
+ * fields and screens permeability.  Every domain starts from rest at the collision's rest
+ * weights with the mean density of the reference's cold start (as mflbm_init_relaxed);
+ * populations are FP32 deviations from that state, the reductions FP64.
+ * ------------------------------------------------------------------------------------- */
+typedef struct mflbm_ensemble mflbm_ensemble;
+
+/* geometries: n_domains voxel arrays of nx*ny*nz bytes (1 solid, 0 pore, x fastest), back
+ * to back; the x = 1 and x = nx planes are made solid.  Reads nx, ny, nz, la_nui1 (inverse
+ * viscosity) and force_z of params.  Per-domain rest densities are written to rho_ref
+ * (n_domains doubles) when it is not null. */
+int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, int32_t n_domains,
+                          double* rho_ref, mflbm_ensemble** out);
+void mflbm_ensemble_destroy(mflbm_ensemble* e);
+const char* mflbm_ensemble_last_error(const mflbm_ensemble* e);
+int64_t mflbm_ensemble_fluid_nodes(const mflbm_ensemble* e);
+
+/* Steps every domain not yet steady; after every `check` steps (even) the relative L2
+ * change of each domain's velocity field since the previous check is computed, and a domain
+ * whose change is below tol stops there.  Returns when all have stopped or after at most
+ * max_steps more steps; a later call continues where this one stopped.  Per domain (arrays
+ * of n_domains, null = skip): steps taken so far (negated while not yet steady);
+ * permeability nu <w> / F over the whole domain at its last check (FP64); the last relative
+ * change.  device_ms (nullable) receives the device time of this call's step kernels. */
+int mflbm_ensemble_solve(mflbm_ensemble* e, int32_t check, double tol, int32_t max_steps, int32_t* steps,
+                         double* permeability, double* change, double* device_ms);
+
+/* Fields of domains first .. first+count-1 at their last check, written to out as
+ * count x 4 x nz x ny x nx floats: u, v, w (compute_macro_vars: j - F/2) and the density
+ * minus the domain's rest density (rho_ref of mflbm_ensemble_create); zero at solid nodes. */
+int mflbm_ensemble_fields(mflbm_ensemble* e, int32_t first, int32_t count, float* out);
+
 #ifdef __cplusplus
 }
 #endif

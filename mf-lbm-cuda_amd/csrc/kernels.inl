@@ -50,46 +50,12 @@ struct W {
 template <typename T>
 __device__ __forceinline__ T wq(int q) { return q == 0 ? W<T>::w0 : (q <= 6 ? W<T>::w1 : W<T>::w2); }
 
-// ---------------------------------------------------------------------------------------
-// Collision + LKR recolouring of one node (src/main_iteration_GPU.cu:116-345).
-// ---------------------------------------------------------------------------------------
-// With ShiftedStorage (FP32) g2 / o2 hold the deviation of the total population from the
-// rest state at rho_ref; the moments are then those of the deviation, in which only
-// m_rho (rho_ref) and m_e (-11 rho_ref) of the rest state differ from zero, so the same
-// relaxation applies with the density deviation in place of the density.
+// MRT collision with Guo forcing of the total populations ft (src/main_iteration_GPU.cu:
+// 164-345): moments, relaxation towards the equilibrium of density m_rho and velocity
+// u = j + F/2, back-transform into f.  With ShiftedStorage ft are deviations from the rest
+// state and m_rho the density deviation (see collide_node).
 template <typename T>
-__device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
-                                          T cnz, T ftmp, T (&o1)[19], T (&o2)[19]) {
-    constexpr bool SH = ShiftedStorage<T>::value;
-    T ft[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) ft[q] = SH ? g2[q] : g1[q] + g2[q];
-    T rho1 = g1[0], rho2 = g2[0];
-#pragma unroll
-    for (int q = 1; q < 19; ++q) { rho1 = rho1 + g1[q]; rho2 = rho2 + g2[q]; }
-    T den, m_rho;
-    if (SH) {
-        m_rho = rho2;  // sum of the deviations
-        den = P.rho_ref + m_rho;
-        // Colour 1 is stored whole, so where fluid 2 is absent its sum carries the float
-        // rounding of O(0.1) populations (~1e-8) against the exact total; left alone, that
-        // residue would read as a trace of fluid 2 and feed the colour gradient.  A
-        // fraction below 1e-6 is taken as none, which keeps such regions at phi = 1
-        // exactly, as colour 2 does on its own where fluid 1 is absent (there colour 1
-        // is exactly zero).
-        if (den - rho1 < T(1e-6) * den) rho1 = den;
-        rho2 = den - rho1;
-    } else {
-        den = rho1 + rho2;
-        m_rho = den;
-    }
-    const T phi = (rho1 - rho2) / (SH ? den : rho1 + rho2);
-
-    // ftmp = 0.5 * gamma * curv * c_norm, evaluated in that order by k_curvature
-    T tmp = ftmp;
-    const T fx = tmp * cnx, fy = tmp * cny, fz = tmp * cnz + P.force_z;
-
-    const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
+__device__ __forceinline__ void mrt_collide(const T (&ft)[19], T m_rho, T fx, T fy, T fz, T omega, T (&f)[19]) {
     const T s_nu = omega, s_e = T(1.19), s_e2 = T(1.4), s_pi = T(1.4), s_q = T(1.2), s_t = T(1.98);
 
     const T ux = ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] + T(0.5) * fx;
@@ -169,7 +135,6 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     sum7 = m_jz + m_qz;
     sum8 = m_3pxx + m_3pixx;
     sum9 = m_pww + m_piww;
-    T f[19];
     f[0] = m_rho - T(30.0) * m_e + T(12.0) * m_e2;
     f[1] = sum1 + m_jx - T(4.0) * m_qx + sum2;
     f[2] = sum1 - m_jx + T(4.0) * m_qx + sum2;
@@ -189,6 +154,50 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     f[16] = sum4 - sum6 + sum7 - sum8 * T(2.0) - m_pyz - m_ty - m_tz;
     f[17] = sum4 + sum6 - sum7 - sum8 * T(2.0) - m_pyz + m_ty + m_tz;
     f[18] = sum4 - sum6 - sum7 - sum8 * T(2.0) + m_pyz - m_ty + m_tz;
+}
+
+// ---------------------------------------------------------------------------------------
+// Collision + LKR recolouring of one node (src/main_iteration_GPU.cu:116-345).
+// ---------------------------------------------------------------------------------------
+// With ShiftedStorage (FP32) g2 / o2 hold the deviation of the total population from the
+// rest state at rho_ref; the moments are then those of the deviation, in which only
+// m_rho (rho_ref) and m_e (-11 rho_ref) of the rest state differ from zero, so the same
+// relaxation applies with the density deviation in place of the density.
+template <typename T>
+__device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
+                                          T cnz, T ftmp, T (&o1)[19], T (&o2)[19]) {
+    constexpr bool SH = ShiftedStorage<T>::value;
+    T ft[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) ft[q] = SH ? g2[q] : g1[q] + g2[q];
+    T rho1 = g1[0], rho2 = g2[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) { rho1 = rho1 + g1[q]; rho2 = rho2 + g2[q]; }
+    T den, m_rho;
+    if (SH) {
+        m_rho = rho2;  // sum of the deviations
+        den = P.rho_ref + m_rho;
+        // Colour 1 is stored whole, so where fluid 2 is absent its sum carries the float
+        // rounding of O(0.1) populations (~1e-8) against the exact total; left alone, that
+        // residue would read as a trace of fluid 2 and feed the colour gradient.  A
+        // fraction below 1e-6 is taken as none, which keeps such regions at phi = 1
+        // exactly, as colour 2 does on its own where fluid 1 is absent (there colour 1
+        // is exactly zero).
+        if (den - rho1 < T(1e-6) * den) rho1 = den;
+        rho2 = den - rho1;
+    } else {
+        den = rho1 + rho2;
+        m_rho = den;
+    }
+    const T phi = (rho1 - rho2) / (SH ? den : rho1 + rho2);
+
+    // ftmp = 0.5 * gamma * curv * c_norm, evaluated in that order by k_curvature
+    T tmp = ftmp;
+    const T fx = tmp * cnx, fy = tmp * cny, fz = tmp * cnz + P.force_z;
+
+    const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
+    T f[19];
+    mrt_collide<T>(ft, m_rho, fx, fy, fz, omega, f);
 
     const T tmp1 = rho1 / den;
     o1[0] = tmp1 * (SH ? f[0] + P.rho_ref * T(rest_weight(0)) : f[0]);

@@ -26,7 +26,9 @@ HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv"
 EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_equilibrium",
            "mflbm_init_relaxed",
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
-           "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe"]
+           "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe",
+           "mflbm_ensemble_create", "mflbm_ensemble_destroy", "mflbm_ensemble_last_error",
+           "mflbm_ensemble_fluid_nodes", "mflbm_ensemble_solve", "mflbm_ensemble_fields"]
 
 
 class Params(C.Structure):
@@ -93,6 +95,18 @@ def load():
     L.mflbm_field_size.restype = C.c_int64
     L.mflbm_bandwidth_probe.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     L.mflbm_bandwidth_probe.restype = C.c_int
+    i32p = C.POINTER(C.c_int32)
+    L.mflbm_ensemble_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_int8), C.c_int32, dp, C.POINTER(vp)]
+    L.mflbm_ensemble_create.restype = C.c_int
+    L.mflbm_ensemble_destroy.argtypes = [vp]
+    L.mflbm_ensemble_last_error.argtypes = [vp]
+    L.mflbm_ensemble_last_error.restype = C.c_char_p
+    L.mflbm_ensemble_fluid_nodes.argtypes = [vp]
+    L.mflbm_ensemble_fluid_nodes.restype = C.c_int64
+    L.mflbm_ensemble_solve.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, i32p, dp, dp, dp]
+    L.mflbm_ensemble_solve.restype = C.c_int
+    L.mflbm_ensemble_fields.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+    L.mflbm_ensemble_fields.restype = C.c_int
     for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_init_relaxed", "mflbm_color_gradient",
                  "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro",
                  "mflbm_monitor", "mflbm_sync"):
