@@ -418,6 +418,20 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
             unpack_ijk(lf[t], i, j, k);
             tag[i4(d, i, j, k)] |= (int32_t)(t << 2);
         }
+        // solid-boundary nodes: which D3Q19 neighbours are fluid (the K3/K6 averages)
+        for (int k = -3; k <= d.nz + 4; ++k)
+            for (int j = -3; j <= d.ny + 4; ++j)
+                for (int i = -3; i <= d.nx + 4; ++i) {
+                    const int64_t c = i4(d, i, j, k);
+                    if (tag[c] != TAG_SOLID_BND) continue;
+                    int32_t mask = 0;
+                    for (int q = 1; q < 19; ++q) {
+                        const int a = i + EX[q], b = j + EY[q], e = k + EZ[q];
+                        if (a < -3 || a > d.nx + 4 || b < -3 || b > d.ny + 4 || e < -3 || e > d.nz + 4) continue;
+                        if (tag_fluid(tag[i4(d, a, b, e)] & 3)) mask |= 1 << (q - 1);
+                    }
+                    tag[c] |= mask << 2;
+                }
         HIPCHK(h, upload_i32_raw(&h->tag, tag));
         h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
         h->grad_shape = GRAD_SHAPE_DEFAULT;

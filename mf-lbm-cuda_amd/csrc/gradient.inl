@@ -15,6 +15,11 @@
 // only solid-boundary entries and read only fluid entries, and the D3Q19 neighbours of a
 // fluid node are fluid or solid-boundary nodes by construction of walls_type.
 //
+// Node types never reach LDS: each thread keeps the tag words of the entries it loaded
+// (the same entries it classifies into the stage lists) for the last four planes, and
+// what a stage needs of them travels in its list entry -- the fluid-neighbour mask of a
+// solid-boundary node (K3, K6) and the wetting flag of a fluid node (K4).
+//
 // Every value goes through the same device functions as the split kernels (iso4_at,
 // interface_normal, curvature_of, the K3/K6 loops in q order), so both paths round alike.
 // The phi written back is the solid-boundary phi of the owned planes (plus the inlet ghost
@@ -47,16 +52,17 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H, NP = WX * WY;  // loaded plane
     constexpr int VX = TX + 4, VY = TY + 4, NV = VX * VY;          // normals plane (halo 2)
     constexpr int NO = TX * TY;                                    // owned plane
-    constexpr int NL = (NP + GRAD_THREADS - 1) / GRAD_THREADS;
-    constexpr int SEG = (NP + GRAD_WAVES - 1) / GRAD_WAVES;        // list segment per wave
-    constexpr int NSUB = (SEG + 63) / 64;
+    constexpr int SEG = (NP + GRAD_WAVES - 1) / GRAD_WAVES;        // entries per wave
+    constexpr int NSUB = (SEG + 63) / 64;                          // entries per lane
+    static_assert(NP <= 1024, "list entries keep the plane position in 10 bits");
     __shared__ T s_phi[4][NP];
     __shared__ T s_cn[4][3][NV];
     __shared__ T s_cnorm[3][NO];
-    __shared__ int8_t s_tag[6][NP];
-    // work lists of the four stages (K3, K4, K6, K7), double-buffered by iteration parity:
-    // loaded-plane entries that a stage computes, one segment per wave
-    __shared__ int16_t s_list[2][4][GRAD_WAVES * SEG];
+    // work lists of the four stages, double-buffered by iteration parity: loaded-plane
+    // entries that a stage computes, one segment per wave.  K3 and K6 entries carry the
+    // fluid-neighbour mask (e | mask << 10), K4 entries the wetting flag (e | wet << 14).
+    __shared__ int32_t s_lbnd[2][2][GRAD_WAVES * SEG];  // K3, K6
+    __shared__ int16_t s_lfl[2][2][GRAD_WAVES * SEG];   // K4, K7
     __shared__ int s_cnt[2][4][GRAD_WAVES];
 
     const Dims& d = A.d;
@@ -69,35 +75,53 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     const int nxo = min(TX, d.nx - (ib + H) + 1), nyo = min(TY, d.ny - (jb + H) + 1);  // owned extent
 
     auto s4 = [](int p) { return (p + 8) & 3; };
-    auto s6 = [](int p) { return (p + 12) % 6; };
     auto s3 = [](int p) { return (p + 12) % 3; };
 
+    // Entry of lane `lane`, sub-step sb of this wave: the mapping of the loads and of the
+    // list classification alike, so that a thread classifies the entries it loaded.
+    auto entry = [&](int sb, bool& in) {
+        const int j = sb * 64 + lane, e = wave * SEG + j;
+        in = j < SEG && e < NP;
+        return e;
+    };
     // register staging of the next plane: its loads are in flight while the stages run
     // (kept raw until the store, so that nothing waits for them early)
-    T pv[NL];
-    int32_t tv[NL];
+    T pv[NSUB];
+    int32_t tv[NSUB];
     auto fetch = [&](int p) {
 #pragma unroll
-        for (int m = 0; m < NL; ++m) {
-            const int e = tid + m * GRAD_THREADS;
+        for (int sb = 0; sb < NSUB; ++sb) {
+            bool in;
+            const int e = entry(sb, in);
             const int gi = ib + e % WX, gj = jb + e / WX;
-            if (e < NP && gi <= d.nx + 4 && gj <= d.ny + 4) {
+            if (in && gi <= d.nx + 4 && gj <= d.ny + 4) {
                 const int c = i4(d, gi, gj, p);
-                pv[m] = A.phi[c];
-                tv[m] = A.tag[c];
+                pv[sb] = A.phi[c];
+                tv[sb] = A.tag[c];
             } else {
-                pv[m] = T(0);
-                tv[m] = TAG_SOLID;
+                pv[sb] = T(0);
+                tv[sb] = TAG_SOLID;
             }
         }
     };
-    // i-th entry of a stage list (segments concatenated in wave order)
-    // i-th entry of a stage list; clamped into the plane so that no address formed from
-    // it can leave the arrays (callers only ask for i below the list total)
-    auto item = [&](const int16_t* list, const int* cnt, int i) -> int {
-        const int e = list_item<GRAD_WAVES>(list, cnt, SEG, i);
-        return e < 0 ? 0 : (e >= NP ? NP - 1 : e);
+    // tag words of this thread's entries in planes p-1 .. p-4 (th[0] = p-1)
+    int32_t th[4][NSUB];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int sb = 0; sb < NSUB; ++sb) th[a][sb] = TAG_SOLID;
+    // i-th entry of a stage list (segments concatenated in wave order), clamped into the
+    // list's value range so that no address formed from it can leave the arrays (callers
+    // only ask for i below the list total)
+    auto item16 = [&](const int16_t* list, const int* cnt, int i) -> int {
+        const int v = list_item<GRAD_WAVES>(list, cnt, SEG, i);
+        return v < 0 ? 0 : v;
     };
+    auto item32 = [&](const int32_t* list, const int* cnt, int i) -> int {
+        const int v = list_item<GRAD_WAVES>(list, cnt, SEG, i);
+        return v < 0 ? 0 : v;
+    };
+    auto pos = [](int v) { return min(v & 1023, NP - 1); };
     auto total = [](const int* cnt) {
         int t = 0;
 #pragma unroll
@@ -106,53 +130,60 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     };
 
     fetch(k0 - H);
+    int32_t tcur[NSUB];
+#pragma unroll
+    for (int sb = 0; sb < NSUB; ++sb) tcur[sb] = TAG_SOLID;
     for (int p = k0 - H; p <= k1 + H; ++p) {
         const int par = p & 1;
+#pragma unroll
+        for (int sb = 0; sb < NSUB; ++sb) {
+            th[3][sb] = th[2][sb];
+            th[2][sb] = th[1][sb];
+            th[1][sb] = th[0][sb];
+            th[0][sb] = tcur[sb];
+            tcur[sb] = tv[sb];
+        }
         {
             T* ph = s_phi[s4(p)];
-            int8_t* tg = s_tag[s6(p)];
 #pragma unroll
-            for (int m = 0; m < NL; ++m) {
-                const int e = tid + m * GRAD_THREADS;
-                if (e < NP) { ph[e] = pv[m]; tg[e] = (int8_t)(tv[m] & 3); }
+            for (int sb = 0; sb < NSUB; ++sb) {
+                bool in;
+                const int e = entry(sb, in);
+                if (in) ph[e] = pv[sb];
             }
         }
         if (p < k1 + H) fetch(p + 1);
-        // stage lists of this iteration, from tags loaded in earlier iterations
+        // stage lists of this iteration, from the tags of earlier planes kept in registers
         {
-            const int8_t* t3 = s_tag[s6(p - 1)];
-            const int8_t* t4 = s_tag[s6(p - 2)];
-            const int8_t* t6 = s_tag[s6(p - 3)];
-            const int8_t* t7 = s_tag[s6(p - 4)];
             int c3 = 0, c4 = 0, c6 = 0, c7 = 0;
-            int16_t* l3 = &s_list[par][0][wave * SEG];
-            int16_t* l4 = &s_list[par][1][wave * SEG];
-            int16_t* l6 = &s_list[par][2][wave * SEG];
-            int16_t* l7 = &s_list[par][3][wave * SEG];
+            int32_t* l3 = &s_lbnd[par][0][wave * SEG];
+            int32_t* l6 = &s_lbnd[par][1][wave * SEG];
+            int16_t* l4 = &s_lfl[par][0][wave * SEG];
+            int16_t* l7 = &s_lfl[par][1][wave * SEG];
 #pragma unroll
             for (int sb = 0; sb < NSUB; ++sb) {
-                const int j = sb * 64 + lane;
-                const int e = wave * SEG + j;
-                const bool in = j < SEG && e < NP;
+                bool in;
+                const int e = entry(sb, in);
                 const int x = e % WX, y = e / WX;
                 const bool r3 = in && x >= 1 && x < WX - 1 && y >= 1 && y < WY - 1;
                 const bool r2 = in && x >= 2 && x < WX - 2 && y >= 2 && y < WY - 2;
                 const bool r1 = in && x >= 3 && x < WX - 3 && y >= 3 && y < WY - 3;
                 const bool r0 = in && x >= H && x < H + nxo && y >= H && y < H + nyo;
-                const bool f3 = r3 && t3[e] == TAG_SOLID_BND;
-                const bool f4 = r2 && tag_fluid(t4[e]);
-                const bool f6 = r1 && t6[e] == TAG_SOLID_BND;
-                const bool f7 = r0 && tag_fluid(t7[e]);
-                auto push = [&](bool f, int16_t* l, int& c) {
+                const int t3 = th[0][sb], t4 = th[1][sb], t6 = th[2][sb], t7 = th[3][sb];
+                const bool f3 = r3 && (t3 & 3) == TAG_SOLID_BND;
+                const bool f4 = r2 && tag_fluid(t4 & 3);
+                const bool f6 = r1 && (t6 & 3) == TAG_SOLID_BND;
+                const bool f7 = r0 && tag_fluid(t7 & 3);
+                auto push = [&](bool f, auto* l, int& c, int v) {
                     const uint64_t m = __ballot(f);
-                    const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                    if (f) l[c + pos] = (int16_t)e;
+                    const int at = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (f) l[c + at] = v;
                     c += __popcll(m);
                 };
-                push(f3, l3, c3);
-                push(f4, l4, c4);
-                push(f6, l6, c6);
-                push(f7, l7, c7);
+                push(f3, l3, c3, e | ((t3 >> 2) << 10));
+                push(f4, l4, c4, e | (((t4 & 3) == TAG_WET) << 14));
+                push(f6, l6, c6, e | ((t6 >> 2) << 10));
+                push(f7, l7, c7, e);
             }
             if (lane == 0) {
                 s_cnt[par][0][wave] = c3; s_cnt[par][1][wave] = c4;
@@ -169,8 +200,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         int32_t rec = 0;
         if (p - 2 >= k0 - 2) {
             const int q = p - 2;
-            const int8_t* tc = s_tag[s6(q)];
-            const int16_t* list = s_list[par][1];
+            const int16_t* list = s_lfl[par][0];
             const int* cnt = s_cnt[par][1];
             const int n = total(cnt);
 #pragma unroll
@@ -178,8 +208,8 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 const int i = tid + m * GRAD_THREADS;
                 wn[m][0] = wn[m][1] = wn[m][2] = T(0);
                 if (i < n) {
-                    const int e = item(list, cnt, i);
-                    if (tc[e] == TAG_WET) {
+                    const int v = item16(list, cnt, i), e = pos(v);
+                    if (v & (1 << 14)) {
                         const int c = i4(d, ib + e % WX, jb + e / WX, q);
                         wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
                     }
@@ -189,7 +219,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         if (p - 4 >= k0) {
             const int* cnt = s_cnt[par][3];
             if (tid < total(cnt)) {
-                const int e = item(s_list[par][3], cnt, tid), x = e % WX, y = e / WX;
+                const int e = pos(item16(s_lfl[par][1], cnt, tid)), x = e % WX, y = e / WX;
                 if (x >= H && x < H + nxo && y >= H && y < H + nyo)  // owned (the list holds only such)
                     rec = A.tag[i4(d, ib + x, jb + y, p - 4)] >> 2;
             }
@@ -200,20 +230,20 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int q = p - 1;
             T* pc = s_phi[s4(q)];
             const T* pl[3] = {s_phi[s4(q - 1)], pc, s_phi[s4(q + 1)]};
-            const int8_t* tl[3] = {s_tag[s6(q - 1)], s_tag[s6(q)], s_tag[s6(q + 1)]};
             const bool wb = q >= kw0 && q <= k1;
-            const int16_t* list = s_list[par][0];
+            const int32_t* list = s_lbnd[par][0];
             const int* cnt = s_cnt[par][0];
             const int n = total(cnt);
             for (int i = tid; i < n; i += GRAD_THREADS) {
-                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                const int lv = item32(list, cnt, i), e = pos(lv), x = e % WX, y = e / WX;
+                const uint32_t mask = (uint32_t)lv >> 10;
                 // branch-free sums: a non-fluid neighbour adds +0 * w = +0, which leaves
                 // the (never negative-zero) partial sums unchanged
                 T ps = T(0), ws = T(0);
 #pragma unroll
                 for (int qq = 1; qq < 19; ++qq) {
                     const int o = e + EX[qq] + WX * EY[qq];
-                    const bool f = tag_fluid(tl[1 + EZ[qq]][o]);
+                    const bool f = (mask >> (qq - 1)) & 1u;
                     ps += keep_if(f, pl[1 + EZ[qq]][o]) * wq<T>(qq);
                     ws += keep_if(f, wq<T>(qq));
                 }
@@ -228,17 +258,16 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         if (p - 2 >= k0 - 2) {
             const int q = p - 2;
             const T* pl[3] = {s_phi[s4(q - 1)], s_phi[s4(q)], s_phi[s4(q + 1)]};
-            const int8_t* tc = s_tag[s6(q)];
             T* cn = &s_cn[s4(q)][0][0];
             T* cno = s_cnorm[s3(q)];
-            const int16_t* list = s_list[par][1];
+            const int16_t* list = s_lfl[par][0];
             const int* cnt = s_cnt[par][1];
             const int n = total(cnt);
 #pragma unroll
             for (int m = 0; m < N4; ++m) {
                 const int i = tid + m * GRAD_THREADS;
                 if (i >= n) break;
-                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                const int v = item16(list, cnt, i), e = pos(v), x = e % WX, y = e / WX;
                 const int r = (x - 2) + VX * (y - 2);
                 auto a = [&](int dx, int dy, int dz) -> T { return pl[1 + dz][e + dx + WX * dy]; };
                 const T cx = iso4_at<T, 0>(a);
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 const T cz = iso4_at<T, 2>(a);
                 T vx, vy, vz;
                 const T cnorm = interface_normal<T>(
-                    cx, cy, cz, tc[e] == TAG_WET, P.cos_theta,
+                    cx, cy, cz, (v & (1 << 14)) != 0, P.cos_theta,
                     [&](T& nwx, T& nwy, T& nwz) { nwx = wn[m][0]; nwy = wn[m][1]; nwz = wn[m][2]; },
                     vx, vy, vz);
                 cn[r] = vx; cn[NV + r] = vy; cn[2 * NV + r] = vz;
@@ -259,18 +288,18 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         if (p - 3 >= k0 - 1) {
             const int q = p - 3;
             const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
-            const int8_t* tl[3] = {s_tag[s6(q - 1)], s_tag[s6(q)], s_tag[s6(q + 1)]};
             T* cn = &s_cn[s4(q)][0][0];
-            const int16_t* list = s_list[par][2];
+            const int32_t* list = s_lbnd[par][1];
             const int* cnt = s_cnt[par][2];
             const int n = total(cnt);
             for (int i = tid; i < n; i += GRAD_THREADS) {
-                const int e = item(list, cnt, i), x = e % WX, y = e / WX;
+                const int lv = item32(list, cnt, i), e = pos(lv), x = e % WX, y = e / WX;
+                const uint32_t mask = (uint32_t)lv >> 10;
                 const int v = (x - 2) + VX * (y - 2);
                 T sx = T(0), sy = T(0), sz = T(0), ws = T(0);
 #pragma unroll
                 for (int qq = 1; qq < 19; ++qq) {
-                    const bool f = tag_fluid(tl[1 + EZ[qq]][e + EX[qq] + WX * EY[qq]]);
+                    const bool f = (mask >> (qq - 1)) & 1u;
                     const T* c = cl[1 + EZ[qq]];
                     const int o = v + EX[qq] + VX * EY[qq];
                     sx += keep_if(f, c[o]) * wq<T>(qq);
@@ -289,12 +318,12 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int q = p - 4;
             const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
             const T* cno = s_cnorm[s3(q)];
-            const int16_t* list = s_list[par][3];
+            const int16_t* list = s_lfl[par][1];
             const int* cnt = s_cnt[par][3];
             const int n = total(cnt);
             static_assert(NO <= GRAD_THREADS, "one owned node per thread");
             if (tid < n) {
-                const int e = item(list, cnt, tid), x = e % WX, y = e / WX;
+                const int e = pos(item16(list, cnt, tid)), x = e % WX, y = e / WX;
                 const int v = (x - 2) + VX * (y - 2);
                 auto comp = [&](int c) {
                     return [&, c](int dx, int dy, int dz) -> T { return cl[1 + dz][c * NV + v + dx + VX * dy]; };

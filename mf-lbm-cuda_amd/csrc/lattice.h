@@ -115,8 +115,9 @@ struct Sparse {
 
 // Inputs of the fused gradient-chain kernel (gradient.inl).  `tag` holds, for every node of
 // the four-ghost-layer grid, (t << 2) | (walls_type & 3) with walls_type 0 -> 0, 1 -> 1,
-// 2 -> 2, -1 -> 3, and t the fluid-list index (force-record index) of an interior fluid
-// node (0 elsewhere).
+// 2 -> 2, -1 -> 3, where t is the fluid-list index (force-record index) of an interior
+// fluid node, the fluid-neighbour mask of a solid-boundary node (bit q-1 set when the D3Q19
+// neighbour q is a fluid node, walls_type 0 or -1; static, like the geometry), 0 elsewhere.
 template <typename T>
 struct GradArgs {
     Dims d;
@@ -141,8 +142,8 @@ constexpr int GRAD_SHAPE_DEFAULT = 0;
 // w (capacity seg) and records its count cnt[w].  Item i of the concatenated list lies in
 // the last segment whose running prefix does not exceed i.  Returns the entry, or -1 for
 // i outside the list.
-template <int WAVES>
-MFLBM_HD int list_item(const int16_t* list, const int* cnt, int seg, int i) {
+template <int WAVES, typename L>
+MFLBM_HD int list_item(const L* list, const int* cnt, int seg, int i) {
     int sg = 0, base = 0, acc = 0;
 #pragma unroll
     for (int w = 0; w < WAVES - 1; ++w) {
@@ -153,7 +154,7 @@ MFLBM_HD int list_item(const int16_t* list, const int* cnt, int seg, int i) {
     }
     const int k = i - base;
     if (i < 0 || k >= cnt[sg]) return -1;
-    return list[sg * seg + k];
+    return (int)list[sg * seg + k];
 }
 
 constexpr int TAG_FLUID = 0, TAG_SOLID = 1, TAG_SOLID_BND = 2, TAG_WET = 3;

@@ -173,7 +173,6 @@ void check_control(const Control& c) {
     if (c.change_inlet_phase != 0) err.push_back("change_inlet_fluid_phase_cmd != 0 is not built");
     if (c.extreme_large != 0)
         err.push_back("Extreme large simulation domain (extreme_large_sim_cmd = 1) is not supported yet");
-    if (c.steady_option == 1) err.push_back("steady_state_option 1 (capillary pressure) is not built");
     if (c.option < 1 || c.option > 5) err.push_back("initial_fluid_distribution_option must be 1-5");
     if (c.ntime_monitor <= 0 || c.ntime_clock_sum <= 0 || c.ntime_display <= 0 || c.ntime_animation <= 0 ||
         c.ntime_visual <= 0 || c.monitor_profile_ratio <= 0)
@@ -342,6 +341,7 @@ private:
     void save_checkpoint(int where);
     void monitor();
     void monitor_phase_field();
+    void monitor_capillary_pressure();
     void monitor_breakthrough();
     void vtk(int type);
     void vtk_walls() const;
@@ -601,6 +601,52 @@ void Simulation::monitor_phase_field() {
     }
 }
 
+// Steady state on the relative change of the capillary pressure, the difference of the
+// mean densities of the bulk phases over 3 (src/Monitor.cpp:354-445).
+void Simulation::monitor_capillary_pressure() {
+    std::vector<double> rho((size_t)s1_), u((size_t)s1_), v((size_t)s1_), w((size_t)s1_);
+    check(mflbm_macro(h_, rho.data(), u.data(), v.data(), w.data()), "macro");
+    const std::vector<double> phi = download_real(MFLBM_PHI, s4_);
+    double umax = 0.0, pre_w = 0.0, pre_nw = 0.0;
+    long long n_w = 0, n_nw = 0;
+    for (int64_t k = 1; k <= nz_; ++k)
+        for (int64_t j = 1; j <= ny_; ++j)
+            for (int64_t i = 1; i <= nx_; ++i) {
+                const int wall = walls_[i_s2(i, j, k)];
+                const int64_t c1 = i_s1(i, j, k);
+                const double t = (u[c1] * u[c1] + v[c1] * v[c1] + w[c1] * w[c1]) * (1 - wall);
+                if (umax < t) umax = t;
+                if (wall == 0) {
+                    const double ph = phi[i_s4(i, j, k)];
+                    if (ph < -0.99) { pre_w += rho[c1]; n_w++; }
+                    if (ph > 0.99) { pre_nw += rho[c1]; n_nw++; }
+                }
+            }
+    const double eps = 1.1920928955078125e-07;
+    umax = std::sqrt(umax);
+    pre_w = pre_w / ((double)n_w + eps);
+    pre_nw = pre_nw / ((double)n_nw + eps);
+    const double pc = (pre_nw - pre_w) / 3.0;
+    monitor_previous_ = monitor_current_;
+    monitor_current_ = pc;
+    const double rel = std::fabs(monitor_current_ - monitor_previous_) / (std::fabs(monitor_current_) + eps);
+    series("steady_monitor_capillary_pressure_error.dat") << ntime_ << " " << rel << " " << pc << " " << pre_w << " "
+                                                          << pre_nw << " " << rel << " " << umax << std::endl;
+    if (std::isnan(umax) || std::isnan(pc)) {
+        std::cout << "Simulation failed due to NAN!" << std::endl;
+        end_ = 3;
+    } else if (umax < 0.5) {
+        if (rel < c_.convergence && ntime_ > c_.ntime_monitor) {
+            std::cout << "Simulation converged based on change of capillary pressure! Relative error =   " << rel
+                      << " ; maximum velocity = " << umax << std::endl;
+            end_ = 1;
+        }
+    } else {
+        std::cout << "maximum velocity larger than 0.5, simulation failed!!!" << std::endl;
+        end_ = 3;
+    }
+}
+
 // Fluid 1 seen at the plane next to the outlet (src/Monitor.cpp:447-470).
 void Simulation::monitor_breakthrough() {
     const std::vector<double> phi = download_real(MFLBM_PHI, s4_);
@@ -754,6 +800,8 @@ int Simulation::run(const std::string& job_status) {
             if (c_.steady_option == 0 || c_.steady_option == 3) {
                 monitor();
                 if (c_.breakthrough_check == 1) monitor_breakthrough();
+            } else if (c_.steady_option == 1) {
+                monitor_capillary_pressure();
             } else if (c_.steady_option == 2) {
                 monitor_phase_field();
             }

@@ -121,3 +121,24 @@ def test_driver_fp32_run_and_cross_precision_restart(tmp_path):
     assert "Simulation ended after\t200 iterations" in out
     sat = np.loadtxt(os.path.join(root, "results/out1.output/saturation.dat"))
     assert np.isfinite(sat).all() and sat[-1, 0] == 200
+
+
+@pytest.mark.parametrize("option, series", [(1, "steady_monitor_capillary_pressure_error.dat"),
+                                            (2, "steady_monitor_max_phi_change.dat")])
+def test_driver_steady_state_monitors(tmp_path, option, series):
+    """A droplet at rest in a periodic box relaxes to a steady state that the capillary-
+    pressure (1) and phase-field (2) monitors detect."""
+    root = str(tmp_path)
+    n = (24, 24, 24)
+    ctrl = cases.periodic_control(n=n, body_force=0.0, option=5, interface_z0=6.0)
+    ctrl.update(steady_state_option=option, convergence_criteria=1e-3, max_time_step=20000, monitor_timer=200,
+                monitor_profile_timer_ratio=100, ntime_visual=100000, ntime_animation=100000,
+                computation_time_timer=1000, display_steps_timer=1000, n_exclude_inlet=0, n_exclude_outlet=0,
+                checkpoint_save_timer=1e3, checkpoint_2rd_save_timer=1e3, simulation_duration_timer=1e3)
+    write_case(root, ctrl, np.zeros((24, 24, 24), np.int8))
+    say(f"driver: droplet, steady_state_option {option}")
+    out = run_driver(root)
+    assert "converged" in out, out[-1500:]
+    assert open(os.path.join(root, "job_status.txt")).read() == "simulation_done\n"
+    rows = np.loadtxt(os.path.join(root, "results/out1.output", series), ndmin=2)
+    assert rows[-1, 1] < 1e-3 and rows[-1, 0] < 20000
