@@ -64,6 +64,8 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     __shared__ int32_t s_lbnd[2][2][GRAD_WAVES * SEG];  // K3, K6
     __shared__ int16_t s_lfl[2][2][GRAD_WAVES * SEG];   // K4, K7
     __shared__ int s_cnt[2][4][GRAD_WAVES];
+    // 1 when K4 found a non-zero normal anywhere in the normals region of the plane
+    __shared__ int s_any[4];
 
     const Dims& d = A.d;
     const int b = xcd_block();
@@ -153,6 +155,9 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             }
         }
         if (p < k1 + H) fetch(p + 1);
+        // the normals slot K4 fills in this iteration held plane p-6, which no stage reads
+        // any more; its flag starts over
+        if (tid == 0) s_any[s4(p - 2)] = 0;
         // stage lists of this iteration, from the tags of earlier planes kept in registers
         {
             int c3 = 0, c4 = 0, c6 = 0, c7 = 0;
@@ -280,6 +285,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                     vx, vy, vz);
                 cn[r] = vx; cn[NV + r] = vy; cn[2 * NV + r] = vz;
                 if (x >= H && x < H + TX && y >= H && y < H + TY) cno[(x - H) + TX * (y - H)] = cnorm;
+                if (cnorm != T(0)) s_any[s4(q)] = 1;
             }
         }
         lds_barrier();
@@ -292,10 +298,16 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int32_t* list = s_lbnd[par][1];
             const int* cnt = s_cnt[par][2];
             const int n = total(cnt);
+            // with no interface normal in planes q-1 .. q+1 every average below is +0 / ws
+            const bool any = (s_any[s4(q - 1)] | s_any[s4(q)] | s_any[s4(q + 1)]) != 0;
             for (int i = tid; i < n; i += GRAD_THREADS) {
                 const int lv = item32(list, cnt, i), e = pos(lv), x = e % WX, y = e / WX;
                 const uint32_t mask = (uint32_t)lv >> 10;
                 const int v = (x - 2) + VX * (y - 2);
+                if (!any) {
+                    cn[v] = T(0); cn[NV + v] = T(0); cn[2 * NV + v] = T(0);
+                    continue;
+                }
                 T sx = T(0), sy = T(0), sz = T(0), ws = T(0);
 #pragma unroll
                 for (int qq = 1; qq < 19; ++qq) {
@@ -328,12 +340,16 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 auto comp = [&](int c) {
                     return [&, c](int dx, int dy, int dz) -> T { return cl[1 + dz][c * NV + v + dx + VX * dy]; };
                 };
-                const T curv = curvature_of<T>(comp(0), comp(1), comp(2));
+                // away from interfaces c_norm and the normals are zero and so is the force
+                // record, whatever the curvature: a wave with no interface node skips it
+                const T c_norm = cno[(x - H) + TX * (y - H)];
+                T curv = T(0);
+                if (c_norm != T(0)) curv = curvature_of<T>(comp(0), comp(1), comp(2));
                 T* a = A.aux + 4 * (int64_t)rec;
                 a[0] = cl[1][v];
                 a[1] = cl[1][NV + v];
                 a[2] = cl[1][2 * NV + v];
-                a[3] = T(0.5) * P.gamma * curv * cno[(x - H) + TX * (y - H)];
+                a[3] = T(0.5) * P.gamma * curv * c_norm;
             }
         }
     }
