@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cubic lattice edge")
     ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--late-steps", type=int, default=4000,
+                    help="further drainage steps before a second, later measurement (0 = skip)")
     ap.add_argument("--probe-mib", type=int, default=1024, help="plain-copy calibration size (0 = skip)")
     ap.add_argument("--traffic", action=argparse.BooleanOptionalAction, default=True,
                     help="measure HBM bytes per kernel with two rocprofv3 --pmc passes")
@@ -264,6 +266,29 @@ def main():
     log(f"collide odd {odd_ms:.3f} ms, even {even_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), "
         f"boundary {bnd_ms:.3f} ms, gradient {grad_ms:.3f} ms, step (kernels) {step_ms:.3f} ms")
 
+    # Later in the drainage: the gradient chain skips work away from interfaces, so its cost
+    # grows with the invaded volume.  Advance further and measure again.
+    late = None
+    if args.late_steps > 0:
+        t_late = ntime0 + args.steps + args.profile_steps
+        sim.run(t_late, args.late_steps)
+        t_late += args.late_steps
+        nl = 200
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.run(t_late, nl)
+        sim.sync()
+        dl = time.perf_counter() - t0
+        kl = sim.profile(t_late + nl, args.profile_steps)
+        nls = max(1, kl["n_odd"] + kl["n_even"])
+        late = {"after_step": t_late - 1, "ms_per_step": round(dl / nl * 1e3, 4),
+                "mlups": round(n_all * nl / dl / 1e6, 3),
+                "kernel_ms": {"collide_odd": round(kl["collide_odd_ms"] / max(1, kl["n_odd"]), 4),
+                              "collide_even": round(kl["collide_even_ms"] / max(1, kl["n_even"]), 4),
+                              "boundary": round(kl["boundary_ms"] / nls, 4),
+                              "gradient": round(kl["gradient_ms"] / nls, 4)}}
+        log(f"after step {t_late - 1}: {late['mlups']:.1f} MLUPS, gradient {late['kernel_ms']['gradient']:.3f} ms")
+
     # the same kernels on a colour field with no uniform region (random phi in [-1, 1] on
     # every node, populations re-initialised from it): the gradient chain's worst case,
     # where no plane takes the exact uniform-region shortcut
@@ -314,6 +339,7 @@ def main():
         "step_roofline_frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {"collide_odd": round(odd_ms, 4), "collide_even": round(even_ms, 4),
                       "boundary": round(bnd_ms, 4), "gradient": round(grad_ms, 4), "step": round(step_ms, 4)},
+        "late": late,
         "kernel_ms_mixed_field": mixed,
         "traffic_bytes": traffic,
         "copy_probe": probe,
