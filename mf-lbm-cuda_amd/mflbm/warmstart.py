@@ -185,9 +185,15 @@ def generate(seeds, n=64, log=print, chunk=512):
 
 
 class WarmStart:
-    def __init__(self, widths=(16, 32, 64, 128), seed=0):
+    """rho_weight: weight of the density channel in the loss relative to the velocity ones
+    (the pressure field is what shortens the start: on the FP64 oracle the exact steady
+    velocity alone gains nothing over the relaxed start, the exact density alone most of
+    what both give)."""
+
+    def __init__(self, widths=(16, 32, 64, 128), seed=0, rho_weight=16.0):
         self.net = nn.UNet(widths=widths, cin=8, cout=8, n_out=4, seed=seed)
         self.scale = np.ones(4)
+        self.rho_weight = rho_weight
 
     def _tensors(self, geo, target=None):
         x = nn.DevArray.from_numpy(features(geo), "bf16")
@@ -203,6 +209,7 @@ class WarmStart:
         self.scale = np.array([np.mean([np.abs(t[c][f]).mean() for (_, t, _, _), f in zip(data, fl)])
                                for c in range(3)] + [np.mean([t[3][f].std() for (_, t, _, _), f in zip(data, fl)])])
         self.scale[:3] = self.scale[2]  # one velocity scale, set by the flow-direction mean
+        self.scale[3] /= np.sqrt(self.rho_weight)
         self.scale = np.maximum(self.scale, 1e-12)
         tensors = [self._tensors(g, t) for g, t, _, _ in data]
         hist = []
@@ -243,11 +250,20 @@ def evaluate(ws, seeds, n=64, precision=gpu.FP32, log=print):
         row = dict(meta)
         sim = gpu.Solver(params, geo, gpu.FP64)
         t0 = time.perf_counter()
-        row["cold_steps"], cold, _ = run_to_steady(sim, params)
+        row["cold_steps"], cold, rho_cold = run_to_steady(sim, params)
         row["cold_s"] = time.perf_counter() - t0
         sim.close()
         k_cold = permeability(cold, params)
         row["k_cold"] = k_cold
+        # accuracy of the predicted fields against the steady state (fluid nodes)
+        fluid = ~solid_with_walls(geo)
+        pr = [a[1:-1, 1:-1, 1:-1] for a in ws.predict(geo, params)]
+        vel_p = np.stack(pr[1:])
+        vel_p[2] += 0.5 * BODY_FORCE * fluid
+        row["velocity_rel_err"] = float(np.linalg.norm((vel_p - cold)[:, fluid]) / np.linalg.norm(cold[:, fluid]))
+        d_ref = rho_cold[fluid] - rho_cold[fluid].mean()
+        d_pre = pr[0][fluid] - pr[0][fluid].mean()
+        row["density_rel_err"] = float(np.linalg.norm(d_pre - d_ref) / max(np.linalg.norm(d_ref), 1e-300))
         sim = gpu.Solver(params, geo, precision)
         for name in ("relaxed", "warm"):
             t0 = time.perf_counter()
@@ -259,7 +275,8 @@ def evaluate(ws, seeds, n=64, precision=gpu.FP32, log=print):
             row[f"k_{name}_rel_diff"] = abs(row[f"k_{name}"] - k_cold) / abs(k_cold)
         sim.close()
         rows.append(row)
-        log(f"held-out {s}: cold {row['cold_steps']} steps {row['cold_s']:.2f} s, relaxed {row['relaxed_steps']} "
+        log(f"held-out {s}: field errors u {row['velocity_rel_err']:.3f} rho {row['density_rel_err']:.3f}; "
+            f"cold {row['cold_steps']} steps {row['cold_s']:.2f} s, relaxed {row['relaxed_steps']} "
             f"steps {row['relaxed_s']:.2f} s, warm {row['warm_steps']} steps {row['warm_s']:.2f} s; k {k_cold:.4f} / "
             f"{row['k_relaxed']:.4f} / {row['k_warm']:.4f}")
     return rows
@@ -273,7 +290,9 @@ def summarise(rows):
             "warm_vs_cold_wall": tot["warm_s"] / tot["cold_s"], "relaxed_vs_cold_wall": tot["relaxed_s"] / tot["cold_s"],
             "warm_vs_relaxed_steps": tot["warm_steps"] / tot["relaxed_steps"],
             "max_k_warm_rel_diff": float(max(r["k_warm_rel_diff"] for r in rows)),
-            "max_k_relaxed_rel_diff": float(max(r["k_relaxed_rel_diff"] for r in rows))}
+            "max_k_relaxed_rel_diff": float(max(r["k_relaxed_rel_diff"] for r in rows)),
+            "mean_velocity_rel_err": float(np.mean([r["velocity_rel_err"] for r in rows])),
+            "mean_density_rel_err": float(np.mean([r["density_rel_err"] for r in rows]))}
 
 
 def main(argv=None):
@@ -284,17 +303,32 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32", help="of the relaxed and warm solves")
+    ap.add_argument("--rho-weight", type=float, default=16.0, help="loss weight of the density channel")
     ap.add_argument("--out", default=None, help="write the JSON summary here as well")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     data = generate(range(args.train), args.n, log)
     t_data = time.perf_counter() - t0
-    ws = WarmStart()
+    ws = WarmStart(rho_weight=args.rho_weight)
     log(f"network parameters: {ws.net.n_params()}")
     t0 = time.perf_counter()
     hist = ws.train(data, args.epochs, args.lr, log)
     t_train = time.perf_counter() - t0
+    # fit on a few training samples, in the same terms as the held-out field errors
+    fit = []
+    for geo, target, params, _ in data[:4]:
+        fluid = ~solid_with_walls(geo)
+        pr = [a[1:-1, 1:-1, 1:-1] for a in ws.predict(geo, params)]
+        nu = 1.0 / params["la_nui1"]
+        vel = target[:3] * BODY_FORCE / nu
+        vel_p = np.stack(pr[1:])
+        vel_p[2] += 0.5 * BODY_FORCE * fluid
+        d_ref = target[3][fluid] * 3.0 * BODY_FORCE
+        d_pre = pr[0][fluid] - pr[0][fluid].mean()
+        fit.append((float(np.linalg.norm((vel_p - vel)[:, fluid]) / np.linalg.norm(vel[:, fluid])),
+                    float(np.linalg.norm(d_pre - d_ref) / np.linalg.norm(d_ref))))
+    log("training-sample field errors (u, rho): " + ", ".join(f"({a:.3f}, {b:.3f})" for a, b in fit))
     prec = gpu.FP32 if args.precision == "fp32" else gpu.FP64
     rows = evaluate(ws, range(1000, 1000 + args.eval), args.n, prec, log)
     res = {"n": args.n, "train_samples": args.train, "epochs": args.epochs, "params": ws.net.n_params(),
