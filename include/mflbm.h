@@ -37,11 +37,15 @@ typedef struct mflbm_params {
     int64_t nx, ny, nz;
     int32_t iper, jper, kper;
     int32_t wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max;
-    int32_t inlet_bc, outlet_bc; /* 2 = Zou-He pressure; 1 = velocity/convective (not built) */
+    int32_t inlet_bc, outlet_bc; /* inlet: 1 velocity, 2 Zou-He pressure; outlet: 1 convective, 2 Zou-He */
     double lbm_gamma, force_z, la_nui1, la_nui2, lbm_beta, phi_inlet, sa_inject, cos_theta;
     double rho_in, rho_out, relaxation;
     int32_t precision;           /* enum mflbm_precision */
     int32_t flags;               /* MFLBM_FLAG_* */
+    int32_t porous_plate;        /* porous_plate_cmd: 0 none, 1 blocks fluid 1, 2 blocks fluid 2 */
+    int32_t z_porous_plate;      /* Z_porous_plate, 1 .. nz */
+    double uin_avg;              /* mean inlet velocity (velocity inlet, convective outlet):
+                                  * capillary_number * surface_tension / fluid1_viscosity */
 } mflbm_params;
 
 /* Run the colour-gradient chain as four kernels per step, one per reference kernel
@@ -59,7 +63,11 @@ enum mflbm_field {
     MFLBM_CURV = 6,    /* s1 */
     MFLBM_S_NX = 7, MFLBM_S_NY = 8, MFLBM_S_NZ = 9, /* s4 */
     MFLBM_WALLS = 10,      /* s2, int8 0/1 */
-    MFLBM_WALLS_TYPE = 11  /* s4, int8 -1/0/1/2 */
+    MFLBM_WALLS_TYPE = 11, /* s4, int8 -1/0/1/2 */
+    MFLBM_W_IN = 12,       /* (nx+2)(ny+2): inlet velocity profile of the velocity inlet (the
+                            * reference's W_in plane k = 0); the duct profile at create */
+    MFLBM_CONVEC = 13      /* 39 (nx+2)(ny+2): convective-outlet state, f_convec then g_convec
+                            * ([e][y][x] each) then phi_convec, the checkpoint's tail */
 };
 
 /* Monitor sums (src/Monitor.cpp:17-80), accumulated in FP64 on the device for both

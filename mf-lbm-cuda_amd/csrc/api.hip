@@ -11,6 +11,7 @@
 // are stream-ordered and replayed from a captured graph.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -48,6 +49,11 @@ struct mflbm_handle {
     int32_t* pairs[2] = {nullptr, nullptr};
     int npairs[2] = {0, 0};
     int32_t* zmap = nullptr;
+    int32_t* pmap = nullptr;    // porous-plate plane map
+    int in_bc = 0, out_bc = 0;  // open z: 1 velocity inlet / convective outlet, 2 Zou-He
+    int plate_z = 0, plate_blocked = 0;
+    void* w_in = nullptr;       // X1 * Y1 inlet velocity plane
+    void* conv = nullptr;       // 39 * X1 * Y1 convective-outlet state
     int32_t *ext_node = nullptr, *ext_off = nullptr;
     int n_ext = 0;
     int32_t* zstart = nullptr;
@@ -106,6 +112,7 @@ StepConst<T> consts(const mflbm_handle* h) {
     c.beta = (T)p.lbm_beta; c.phi_inlet = (T)p.phi_inlet; c.sa_inject = (T)p.sa_inject; c.cos_theta = (T)p.cos_theta;
     c.rho_in = (T)p.rho_in; c.rho_out = (T)p.rho_out; c.relaxation = (T)p.relaxation;
     c.rho_ref = (T)h->rho_ref;
+    c.uin_avg = (T)p.uin_avg;
     return c;
 }
 
@@ -157,10 +164,9 @@ void enqueue_step(mflbm_handle* h, bool odd, hipEvent_t* ev) {
         Launch<T>::periodic_pdf(S, h->pairs[odd ? 1 : 0], h->npairs[odd ? 1 : 0], s);
         Launch<T>::periodic_phi(F.phi, h->d, h->p.kper, h->p.jper, s);
     }
-    if (h->open_z) {
-        const int in = h->p.inlet_bc == 2, out = h->p.outlet_bc == 2;
-        if (in || out) Launch<T>::zou_he(S, F, P, h->zmap, in, out, odd, s);
-    }
+    if (h->open_z && (h->in_bc || h->out_bc))
+        Launch<T>::open_bc(S, F, P, h->zmap, h->in_bc, h->out_bc, (const T*)h->w_in, (T*)h->conv, odd, s);
+    if (h->plate_z) Launch<T>::porous_plate(S, h->d, P, h->pmap, h->plate_z, h->plate_blocked, odd, s);
     mark();
     if (h->split_gradient) {
         Launch<T>::phi_extrap(F, L, s);
@@ -222,7 +228,8 @@ void free_all(mflbm_handle* h) {
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out,
-                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->ext_node, h->ext_off, h->zstart,
+                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
+                    h->ext_node, h->ext_off, h->zstart,
                     h->tag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -230,7 +237,8 @@ void free_all(mflbm_handle* h) {
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
     h->cells = h->aux = nullptr;
-    h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->ext_node = h->ext_off = h->zstart = nullptr;
+    h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->pmap = h->ext_node = h->ext_off = h->zstart = nullptr;
+    h->w_in = h->conv = nullptr;
     h->tag = nullptr;
     if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
 }
@@ -250,6 +258,12 @@ int field_info(const mflbm_handle* h, int field, int64_t* count, size_t* elem, v
         case MFLBM_S_NZ: *count = d.S4; *elem = esize(h); *ptr = h->snz; return 0;
         case MFLBM_WALLS: *count = d.S2; *elem = 1; *ptr = h->walls; return 0;
         case MFLBM_WALLS_TYPE: *count = d.S4; *elem = 1; *ptr = h->wtype; return 0;
+        case MFLBM_W_IN:
+            if (!h->w_in) return -1;
+            *count = (int64_t)d.X1 * d.Y1; *elem = esize(h); *ptr = h->w_in; return 0;
+        case MFLBM_CONVEC:
+            if (!h->conv) return -1;
+            *count = (int64_t)39 * d.X1 * d.Y1; *elem = esize(h); *ptr = h->conv; return 0;
         default: return -1;
     }
 }
@@ -292,6 +306,37 @@ __global__ __launch_bounds__(256) void k_probe_copy_blocked(const double4* __res
         if (b + m * 256 < n) dst[b + m * 256] = v[m];
 }
 
+// Inlet velocity plane of the velocity inlet (src/Init_multiphase.cpp:258-281 and the
+// rectangular-duct series of src/Misc.cpp:387-419, 1000 terms): zero on the wall rows, the
+// laminar duct profile of mean velocity uin_avg inside.
+std::vector<double> inlet_velocity_profile(const Dims& d, double uin_avg) {
+    const double pi = 3.14159265358979323846;
+    const int nx = d.nx, ny = d.ny;
+    std::vector<double> w((size_t)d.X1 * d.Y1, 0.0);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            if (i > 1 && i < nx && j > 1 && j < ny) w[(size_t)j * d.X1 + i] = uin_avg;
+    const double la_x = (double)(nx - 1) - 0.5 - 0.5, la_y = (double)(ny - 1) - 0.5 - 0.5;
+    const double a = 0.5 * la_x, b = 0.5 * la_y;
+    const int terms = 1000;
+    double t1 = 0.0;
+    for (int n = 1; n <= terms; n += 2) t1 += std::tanh(0.5 * (double)n * pi * b / a) / std::pow((double)n, 5);
+    double t2 = 1.0 - 192.0 / std::pow(pi, 5) * (a / b) * t1;
+    t2 = -3.0 * uin_avg / (t2 * std::pow(a, 2));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            if (!(i > 1 && i < nx && j > 1 && j < ny)) continue;
+            const double xx = i - 1.5 - a, yy = j - 1.5 - b;
+            double t3 = 0.0;
+            for (int n = 1; n <= terms; n += 2)
+                t3 += std::pow(-1.0, 0.5 * (double)(n - 1)) * std::cos(0.5 * (double)n * pi * xx / a) / std::pow((double)n, 3) *
+                      (1.0 - (std::exp(0.5 * (double)n * pi * (yy - b) / a) + std::exp(0.5 * (double)n * pi * (-yy - b) / a)) /
+                                 (1.0 + std::exp(0.5 * (double)n * pi * (-b - b) / a)));
+            w[(size_t)j * d.X1 + i] = t3 * (-16.0 * t2 * std::pow(a, 2) * std::pow(pi, -3));
+        }
+    return w;
+}
+
 }  // namespace
 
 extern "C" {
@@ -314,8 +359,24 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     if (p.iper != 0 || p.wall_x_min != 1 || p.wall_x_max != 1)
         return fail(h, "x must have walls at both ends and no periodicity (src/IO_multiphase.cpp:210-213)");
     h->open_z = (p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0);
-    if (h->open_z && (p.inlet_bc == 1 || p.outlet_bc == 1))
-        return fail(h, "velocity inlet / convective outlet boundary conditions are not built");
+    if (h->open_z) {
+        if (p.inlet_bc < 0 || p.inlet_bc > 2 || p.outlet_bc < 0 || p.outlet_bc > 2)
+            return fail(h, "inlet_bc / outlet_bc must be 0, 1 or 2");
+        if (p.outlet_bc == 1 && p.inlet_bc == 2)
+            return fail(h, "Inlet pressure + outlet convective BC is not supported (src/IO_multiphase.cpp:226-229)");
+        h->in_bc = p.inlet_bc;
+        h->out_bc = p.outlet_bc;
+    }
+    if (p.porous_plate < 0 || p.porous_plate > 2) return fail(h, "porous_plate must be 0, 1 or 2");
+    // With y periodic and z closed the reference launches the plate kernel on the grid left by the
+    // periodic-y kernel, which covers only the first rows of the plane (src/main_iteration_GPU.cu:1911-1957).
+    if (p.porous_plate && !h->open_z && p.jper)
+        return fail(h, "porous plate needs an open z axis or walls in y (the reference covers only part of the "
+                       "plane when y is periodic and z closed)");
+    if (p.porous_plate && p.z_porous_plate >= 1 && p.z_porous_plate <= p.nz) {
+        h->plate_z = p.z_porous_plate;
+        h->plate_blocked = p.porous_plate - 1;
+    }
     h->d = make_dims((int)p.nx, (int)p.ny, (int)p.nz);
     const Dims& d = h->d;
     if ((double)d.S1 * 38.0 > 2.0e9 * 8.0) return fail(h, "domain too large");
@@ -452,7 +513,7 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     // fluid-compact population cells
     LayoutHost lay;
     std::string lerr;
-    if (!build_layout(d, lf, (int)p.kper, (int)p.jper, h->open_z, p.inlet_bc == 2, p.outlet_bc == 2, lay, lerr))
+    if (!build_layout(d, lf, (int)p.kper, (int)p.jper, h->open_z, h->in_bc, h->out_bc, h->plate_z, lay, lerr))
         return fail(h, "layout: %s", lerr.c_str());
     // slot-array stride: the cells a step uses (fluid nodes + extra cells), rounded up to a
     // 4 KiB multiple plus a skew, so that the 38 streams of one node do not start at the
@@ -481,6 +542,24 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     h->npairs[0] = (int)(lay.pairs[0].size() / 3);
     h->npairs[1] = (int)(lay.pairs[1].size() / 3);
     HIPCHK(h, upload_i32(&h->zmap, lay.zmap));
+    if (h->plate_z) HIPCHK(h, upload_i32(&h->pmap, lay.pmap));
+    {
+        const int64_t P2 = (int64_t)d.X1 * d.Y1;
+        if (h->in_bc == 1) {
+            const std::vector<double> w = inlet_velocity_profile(d, p.uin_avg);
+            HIPCHK(h, hipMalloc(&h->w_in, P2 * es));
+            if (h->prec == MFLBM_FP64) {
+                HIPCHK(h, hipMemcpy(h->w_in, w.data(), P2 * sizeof(double), hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> wf(w.begin(), w.end());
+                HIPCHK(h, hipMemcpy(h->w_in, wf.data(), P2 * sizeof(float), hipMemcpyHostToDevice));
+            }
+        }
+        if (h->out_bc == 1) {
+            HIPCHK(h, hipMalloc(&h->conv, 39 * P2 * es));
+            HIPCHK(h, hipMemsetAsync(h->conv, 0, 39 * P2 * es, h->stream));
+        }
+    }
     HIPCHK(h, upload_i32(&h->zstart, lay.zstart));
 
     // monitor scratch: every z-slice reduced by mon_nsplit blocks
@@ -534,6 +613,10 @@ static int init_populations(mflbm_handle* h, const double* rho, const double* u,
         Launch<float>::init_eq(fields<float>(h), dv[0], dv[1], dv[2], dv[3], relaxed, mean, h->stream);
     }
     cells_transfer(h, false);
+    if (h->conv) {  // the convective outlet starts from the initial outlet plane
+        if (h->prec == MFLBM_FP64) Launch<double>::conv_init(fields<double>(h), 1.0, (double*)h->conv, h->stream);
+        else Launch<float>::conv_init(fields<float>(h), (float)h->rho_ref, (float*)h->conv, h->stream);
+    }
     hipError_t e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(h->stream);
     for (int a = 0; a < 4; ++a)

@@ -22,9 +22,10 @@
 //
 // Every value goes through the same device functions as the split kernels (iso4_at,
 // interface_normal, curvature_of, the K3/K6 loops in q order), so both paths round alike.
-// The phi written back is the solid-boundary phi of the owned planes (plus the inlet ghost
-// plane k = 0 for the first z-run): the only solid-boundary values a later kernel of the
-// step reads (Zou-He inlet/outlet and the periodic copies).  The full dense gradient
+// The phi written back is the solid-boundary phi of the owned planes (plus the ghost plane
+// k = 0 for the first z-run and k = nz + 1 for the last): the only solid-boundary values a
+// later kernel of the step reads (the open inlets and outlets, which keep the ghost phi of
+// wall columns, and the periodic copies).  The full dense gradient
 // fields are produced on request by the split kernels (they are functions of phi alone).
 #pragma once
 #include "kernels.inl"
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     const int tx = b % A.tiles_x, ty = (b / A.tiles_x) % A.tiles_y, tz = b / (A.tiles_x * A.tiles_y);
     const int ib = 1 + tx * TX - H, jb = 1 + ty * TY - H;  // node of loaded-plane entry (0, 0)
     const int k0 = 1 + tz * A.kchunk, k1 = min(d.nz, k0 + A.kchunk - 1);
-    const int kw0 = tz == 0 ? 0 : k0;
+    const int kw0 = tz == 0 ? 0 : k0, kw1 = k1 == d.nz ? d.nz + 1 : k1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nxo = min(TX, d.nx - (ib + H) + 1), nyo = min(TY, d.ny - (jb + H) + 1);  // owned extent
 
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int q = p - 1;
             T* pc = s_phi[s4(q)];
             const T* pl[3] = {s_phi[s4(q - 1)], pc, s_phi[s4(q + 1)]};
-            const bool wb = q >= kw0 && q <= k1;
+            const bool wb = q >= kw0 && q <= kw1;
             const int32_t* list = s_lbnd[par][0];
             const int* cnt = s_cnt[par][0];
             const int n = total(cnt);

@@ -7,7 +7,9 @@
 //   k_collide<ODD>     kernel_odd_color_GPU / kernel_even_color_GPU   (56-388 / 395-726)
 //   k_periodic_pdf     perioidc_BC_{Z,Y,ZY}_{odd,even}                (1529-1689)
 //   k_periodic_phi     periodic_phi_{Z,Y,ZY}                          (1691-1733)
-//   k_zou_he<AFTER>    inlet/outlet_Zou_He_pressure_BC_{before,after}_odd_GPU (1085-1524)
+//   k_open_bc<AFTER>   inlet/outlet_Zou_He_pressure_BC_{before,after}_odd_GPU (1085-1524),
+//                      inlet_bounce_back_velocity_BC_* (1008-1077), outlet_convective_BC_* (1245-1357)
+//   k_porous_plate     porous_plate_BC_{before,after}_odd (1744-1882)
 //   k_phi_extrap       extrapolate_phi_toSolid                        (732-755)
 //   k_normals          normalDirectionsOfInterfaces + alter_color_gradient_solid_surface_GPU
 //                      (757-878), fused: the wetting correction only reads the node's own
@@ -342,34 +344,85 @@ struct ColourRef {
     }
 };
 
-// Zou-He inlet/outlet, one thread per (i, j).  zmap[plane][slot][b * (nx+2) + a] gives the
-// cell of (a, b, k) for the six planes k = 0, 1, 2, nz-1, nz, nz+1.
+// Runs fn(f) with the population accessor of the storage: f(a, b, c, e, g) is a plain
+// reference for the two-colour storage and a ColourRef for the deviation storage; cell_of
+// maps (a, b, c, e) to the cell index.
+template <typename T, class CellOf, class Fn>
+__device__ __forceinline__ void with_cells(const Sparse<T>& S, T rho_ref, CellOf&& cell_of, Fn&& fn) {
+    const int64_t NS = S.ns;
+    if constexpr (ShiftedStorage<T>::value) {
+        auto f = [&](int a, int b, int c, int e, int g) -> ColourRef<T> {
+            const int64_t cell = cell_of(a, b, c, e);
+            return ColourRef<T>{S.f + e * NS + cell, S.f + (e + 19) * NS + cell, rho_ref * (T)rest_weight(e), g == 1};
+        };
+        fn(f);
+    } else {
+        auto f = [&](int a, int b, int c, int e, int g) -> T& { return S.f[(e + 19 * g) * NS + cell_of(a, b, c, e)]; };
+        fn(f);
+    }
+}
+
+// Open inlet/outlet, one thread per (i, j): in_bc / out_bc 2 = Zou-He, 1 = velocity inlet /
+// convective outlet.  zmap[plane][slot][b * (nx+2) + a] gives the cell of (a, b, k) for the
+// six planes k = 0, 1, 2, nz-1, nz, nz+1.  w_in: the inlet velocity plane (X1 * Y1);
+// conv: the convective-outlet state, 38 planes of populations [g][e] then phi.
 template <typename T, bool AFTER_ODD>
-__global__ __launch_bounds__(256) void k_zou_he(Sparse<T> S, Fields<T> F, StepConst<T> P, const int32_t* __restrict__ zmap,
-                                                int do_in, int do_out) {
+__global__ __launch_bounds__(256) void k_open_bc(Sparse<T> S, Fields<T> F, StepConst<T> P, const int32_t* __restrict__ zmap,
+                                                 int in_bc, int out_bc, const T* __restrict__ w_in, T* __restrict__ conv) {
     const Dims& d = F.d;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= d.nx * d.ny) return;
     const int i = t % d.nx + 1, j = t / d.nx + 1, nz = d.nz;
-    const int64_t NS = S.ns;
-    const int64_t plane = (int64_t)19 * d.X1 * d.Y1;
+    const int64_t P2 = (int64_t)d.X1 * d.Y1, plane = 19 * P2;
     auto cell_of = [&](int a, int b, int c, int e) -> int64_t {
         const int pl = c <= 2 ? c : 3 + (c - (nz - 1));
-        return zmap[pl * plane + (int64_t)e * d.X1 * d.Y1 + b * d.X1 + a];
+        return zmap[pl * plane + (int64_t)e * P2 + b * d.X1 + a];
     };
     auto wall = [&](int a, int b, int c) -> int { return F.walls[i2(d, a, b, c)]; };
     auto ph = [&](int a, int b, int c) -> T& { return F.phi[i4(d, a, b, c)]; };
-    if constexpr (ShiftedStorage<T>::value) {
-        auto f = [&](int a, int b, int c, int e, int g) -> ColourRef<T> {
-            const int64_t cell = cell_of(a, b, c, e);
-            return ColourRef<T>{S.f + e * NS + cell, S.f + (e + 19) * NS + cell, P.rho_ref * (T)rest_weight(e),
-                                g == 1};
-        };
-        zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
-    } else {
-        auto f = [&](int a, int b, int c, int e, int g) -> T& { return S.f[(e + 19 * g) * NS + cell_of(a, b, c, e)]; };
-        zou_he_column<T, AFTER_ODD>(P, i, j, nz, do_in, do_out, f, wall, ph);
+    const int64_t col = (int64_t)j * d.X1 + i;
+    with_cells<T>(S, P.rho_ref, cell_of, [&](auto&& f) {
+        if (in_bc == 1) velocity_inlet_column<T, AFTER_ODD>(P, i, j, w_in[col], f, wall, ph);
+        if (in_bc == 2 || out_bc == 2) zou_he_column<T, AFTER_ODD>(P, i, j, nz, in_bc == 2, out_bc == 2, f, wall, ph);
+        if (out_bc == 1) {
+            auto cs = [&](int e, int g) -> T& { return conv[(int64_t)(e + 19 * g) * P2 + col]; };
+            auto cp = [&]() -> T& { return conv[38 * P2 + col]; };
+            convective_outlet_column<T, AFTER_ODD>(P, i, j, nz, f, wall, ph, cs, cp);
+        }
+    });
+}
+
+// Porous plate at k = zp, one thread per (i, j); pmap as zmap for the planes zp-1, zp, zp+1.
+template <typename T, bool AFTER_ODD>
+__global__ __launch_bounds__(256) void k_porous_plate(Sparse<T> S, Dims d, StepConst<T> P, const int32_t* __restrict__ pmap,
+                                                      int zp, int blocked) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= d.nx * d.ny) return;
+    const int i = t % d.nx + 1, j = t / d.nx + 1;
+    const int64_t P2 = (int64_t)d.X1 * d.Y1, plane = 19 * P2;
+    auto cell_of = [&](int a, int b, int c, int e) -> int64_t {
+        return pmap[(c - (zp - 1)) * plane + (int64_t)e * P2 + b * d.X1 + a];
+    };
+    with_cells<T>(S, P.rho_ref, cell_of, [&](auto&& f) { porous_plate_column<AFTER_ODD>(i, j, zp, blocked, f); });
+}
+
+// Convective-outlet state of a new run: the populations and phi of the outlet plane k = nz
+// of the initial state (src/Init_multiphase.cpp:445-494), in plain colour values.
+template <typename T>
+__global__ __launch_bounds__(256) void k_conv_init(Fields<T> F, T rho_ref, T* __restrict__ conv) {
+    const Dims& d = F.d;
+    const int64_t P2 = (int64_t)d.X1 * d.Y1;
+    const int64_t col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= P2) return;
+    const int64_t n = col + P2 * d.nz;  // (a, b, nz) in the s1 grid
+    for (int e = 0; e < 19; ++e) {
+        T g1 = F.pdf[n + d.S1 * e], g2 = F.pdf[n + d.S1 * (e + 19)];
+        if (ShiftedStorage<T>::value) g2 = (g2 + rho_ref * (T)rest_weight(e)) - g1;
+        conv[(int64_t)e * P2 + col] = g1;
+        conv[(int64_t)(e + 19) * P2 + col] = g2;
     }
+    const int a = (int)(col % d.X1), b = (int)(col / d.X1);
+    conv[38 * P2 + col] = F.phi[i4(d, a, b, d.nz)];
 }
 
 // Dense reference layout <-> compact storage.  Fluid cells: t < nf; extra cells: the

@@ -73,6 +73,7 @@ template <typename T>
 struct StepConst {
     T gamma, force_z, nui1, nui2, beta, phi_inlet, sa_inject, cos_theta, rho_in, rho_out, relaxation;
     T rho_ref;  // ShiftedStorage: density of the rest state the deviations are taken from
+    T uin_avg;  // convective-outlet velocity
 };
 
 struct NodeLists {

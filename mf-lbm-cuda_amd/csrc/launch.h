@@ -11,8 +11,11 @@ struct Launch {
     static void collide(const Sparse<T>& S, T* phi, const Dims& d, const StepConst<T>& P, bool odd, hipStream_t s);
     static void periodic_pdf(const Sparse<T>& S, const int32_t* pairs, int npairs, hipStream_t s);
     static void periodic_phi(T* phi, const Dims& d, int kper, int jper, hipStream_t s);
-    static void zou_he(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int do_in,
-                       int do_out, bool after_odd, hipStream_t s);
+    static void open_bc(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int in_bc,
+                        int out_bc, const T* w_in, T* conv, bool after_odd, hipStream_t s);
+    static void porous_plate(const Sparse<T>& S, const Dims& d, const StepConst<T>& P, const int32_t* pmap, int zp,
+                             int blocked, bool after_odd, hipStream_t s);
+    static void conv_init(const Fields<T>& F, T rho_ref, T* conv, hipStream_t s);
     static void phi_extrap(const Fields<T>& F, const NodeLists& L, hipStream_t s);
     static void normals(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, hipStream_t s);
     static void normal_extrap(const Fields<T>& F, const NodeLists& L, hipStream_t s);

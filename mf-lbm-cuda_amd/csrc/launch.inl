@@ -35,11 +35,24 @@ void Launch<T>::periodic_phi(T* phi, const Dims& d, int kper, int jper, hipStrea
 }
 
 template <typename T>
-void Launch<T>::zou_he(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int do_in,
-                       int do_out, bool after_odd, hipStream_t s) {
-    const int64_t n = (int64_t)F.d.nx * F.d.ny;
-    if (after_odd) dev::k_zou_he<T, true><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, do_in, do_out);
-    else dev::k_zou_he<T, false><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, do_in, do_out);
+void Launch<T>::open_bc(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int in_bc,
+                        int out_bc, const T* w_in, T* conv, bool after_odd, hipStream_t s) {
+    const int n = F.d.nx * F.d.ny;
+    if (after_odd) dev::k_open_bc<T, true><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv);
+    else dev::k_open_bc<T, false><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv);
+}
+
+template <typename T>
+void Launch<T>::porous_plate(const Sparse<T>& S, const Dims& d, const StepConst<T>& P, const int32_t* pmap, int zp,
+                             int blocked, bool after_odd, hipStream_t s) {
+    const int n = d.nx * d.ny;
+    if (after_odd) dev::k_porous_plate<T, true><<<blocks(n), 256, 0, s>>>(S, d, P, pmap, zp, blocked);
+    else dev::k_porous_plate<T, false><<<blocks(n), 256, 0, s>>>(S, d, P, pmap, zp, blocked);
+}
+
+template <typename T>
+void Launch<T>::conv_init(const Fields<T>& F, T rho_ref, T* conv, hipStream_t s) {
+    dev::k_conv_init<T><<<blocks((int64_t)F.d.X1 * F.d.Y1), 256, 0, s>>>(F, rho_ref, conv);
 }
 
 template <typename T>

@@ -14,8 +14,9 @@
 
 namespace mflbm {
 
-bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, int jper, int open_z, int zou_in,
-                  int zou_out, LayoutHost& out, std::string& err) {
+bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, int jper, int open_z, int in_bc,
+                  int out_bc, int plate_z, LayoutHost& out, std::string& err) {
+    const int zou_in = open_z && in_bc == 2, zou_out = open_z && out_bc == 2;
     const int64_t S1 = d.S1;
     const int nf = (int)fluid.size();
     out = LayoutHost();
@@ -68,6 +69,38 @@ bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, i
             }
         if (bad) { err = "Zou-He access outside the population array"; return false; }
     }
+    // 4. velocity inlet, convective outlet and porous plate accesses of both variants
+    {
+        StepConst<double> P{};
+        double dummy = 0.0;
+        bool bad = false;
+        auto rec = [&](int a, int b, int c, int e, int) -> double& {
+            if (!note(i1(d, a, b, c), e)) bad = true;
+            return dummy;
+        };
+        auto wall = [&](int, int, int) -> int { return 0; };
+        auto ph = [&](int, int, int) -> double& { return dummy; };
+        auto cs = [&](int, int) -> double& { return dummy; };
+        auto cp = [&]() -> double& { return dummy; };
+        for (int j = 1; j <= d.ny; ++j)
+            for (int i = 1; i <= d.nx; ++i) {
+                if (open_z && in_bc == 1) {
+                    velocity_inlet_column<double, false>(P, i, j, 0.0, rec, wall, ph);
+                    velocity_inlet_column<double, true>(P, i, j, 0.0, rec, wall, ph);
+                }
+                if (open_z && out_bc == 1) {
+                    convective_outlet_column<double, false>(P, i, j, d.nz, rec, wall, ph, cs, cp);
+                    convective_outlet_column<double, true>(P, i, j, d.nz, rec, wall, ph, cs, cp);
+                }
+                if (plate_z >= 1 && plate_z <= d.nz) {
+                    porous_plate_column<false>(i, j, plate_z, 0, rec);
+                    porous_plate_column<true>(i, j, plate_z, 0, rec);
+                    porous_plate_column<false>(i, j, plate_z, 1, rec);
+                    porous_plate_column<true>(i, j, plate_z, 1, rec);
+                }
+            }
+        if (bad) { err = "boundary access outside the population array"; return false; }
+    }
     // extra cells: sorted, unique, per slot
     out.ext_off.assign(20, 0);
     size_t mx = 0;
@@ -117,8 +150,18 @@ bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, i
             out.pairs[odd].push_back(cs);
         }
     }
-    // Zou-He plane maps
-    if (open_z && (zou_in || zou_out)) {
+    // porous-plate plane map
+    if (plate_z >= 1 && plate_z <= d.nz) {
+        const int64_t P2 = (int64_t)d.X1 * d.Y1;
+        out.pmap.assign((size_t)3 * 19 * P2, -1);
+        for (int p = 0; p < 3; ++p)
+            for (int s = 0; s < 19; ++s)
+                for (int b = 0; b < d.Y1; ++b)
+                    for (int a = 0; a < d.X1; ++a)
+                        out.pmap[((size_t)p * 19 + s) * P2 + (size_t)b * d.X1 + a] = cell(i1(d, a, b, plate_z - 1 + p), s);
+    }
+    // open-boundary plane maps
+    if (open_z && (in_bc || out_bc)) {
         const int64_t P2 = (int64_t)d.X1 * d.Y1;
         out.zmap.assign((size_t)6 * 19 * P2, -1);
         const int kk[6] = {0, 1, 2, d.nz - 1, d.nz, d.nz + 1};
@@ -127,6 +170,58 @@ bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, i
                 for (int b = 0; b < d.Y1; ++b)
                     for (int a = 0; a < d.X1; ++a)
                         out.zmap[((size_t)p * 19 + s) * P2 + (size_t)b * d.X1 + a] = cell(i1(d, a, b, kk[p]), s);
+    }
+    // every access of the boundary kernels resolves through its plane map, looked up exactly as
+    // the kernels do (k_open_bc, k_porous_plate), to a cell of the layout
+    {
+        const int64_t P2 = (int64_t)d.X1 * d.Y1;
+        StepConst<double> P{};
+        double dummy = 0.0;
+        bool bad = false;
+        auto zrec = [&](int a, int b, int c, int e, int) -> double& {
+            const int pl = c <= 2 ? c : 3 + (c - (d.nz - 1));
+            if (pl < 0 || pl > 5 || a < 0 || a >= d.X1 || b < 0 || b >= d.Y1 || e < 0 || e >= 19) bad = true;
+            else {
+                const int32_t c2 = out.zmap[((size_t)pl * 19 + e) * P2 + (size_t)b * d.X1 + a];
+                if (c2 < 0 || c2 >= out.ns) bad = true;
+            }
+            return dummy;
+        };
+        auto prec = [&](int a, int b, int c, int e, int) -> double& {
+            const int pl = c - (plate_z - 1);
+            if (pl < 0 || pl > 2 || a < 0 || a >= d.X1 || b < 0 || b >= d.Y1 || e < 0 || e >= 19) bad = true;
+            else {
+                const int32_t c2 = out.pmap[((size_t)pl * 19 + e) * P2 + (size_t)b * d.X1 + a];
+                if (c2 < 0 || c2 >= out.ns) bad = true;
+            }
+            return dummy;
+        };
+        auto wall = [&](int, int, int) -> int { return 0; };
+        auto ph = [&](int, int, int) -> double& { return dummy; };
+        auto cs = [&](int, int) -> double& { return dummy; };
+        auto cp = [&]() -> double& { return dummy; };
+        const bool zmapped = open_z && (in_bc || out_bc);
+        for (int j = 1; j <= d.ny && !bad; ++j)
+            for (int i = 1; i <= d.nx; ++i) {
+                if (zmapped && in_bc == 1) {
+                    velocity_inlet_column<double, false>(P, i, j, 0.0, zrec, wall, ph);
+                    velocity_inlet_column<double, true>(P, i, j, 0.0, zrec, wall, ph);
+                }
+                if (zmapped && (zou_in || zou_out)) {
+                    zou_he_column<double, false>(P, i, j, d.nz, zou_in, zou_out, zrec, wall, ph);
+                    zou_he_column<double, true>(P, i, j, d.nz, zou_in, zou_out, zrec, wall, ph);
+                }
+                if (zmapped && out_bc == 1) {
+                    convective_outlet_column<double, false>(P, i, j, d.nz, zrec, wall, ph, cs, cp);
+                    convective_outlet_column<double, true>(P, i, j, d.nz, zrec, wall, ph, cs, cp);
+                }
+                if (plate_z >= 1 && plate_z <= d.nz)
+                    for (int blocked = 0; blocked < 2; ++blocked) {
+                        porous_plate_column<false>(i, j, plate_z, blocked, prec);
+                        porous_plate_column<true>(i, j, plate_z, blocked, prec);
+                    }
+            }
+        if (bad) { err = "boundary access without a cell in its plane map"; return false; }
     }
     // slice offsets of the fluid list
     out.zstart.assign(d.nz + 1, 0);

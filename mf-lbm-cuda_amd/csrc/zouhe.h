@@ -1,5 +1,6 @@
-// Zou-He pressure inlet (k = 1) and outlet (k = nz) for one (i, j) column
-// (src/main_iteration_GPU.cu:1085-1241 inlet, 1363-1524 outlet).
+// Open-boundary column functions: Zou-He pressure inlet (k = 1) and outlet (k = nz) for
+// one (i, j) column (src/main_iteration_GPU.cu:1085-1241 inlet, 1363-1524 outlet), the
+// velocity inlet, the convective outlet and the porous plate.
 //
 // Written against three accessors so the same body serves the device kernel and the
 // host-side pass that records which population slots the boundary touches:
@@ -119,6 +120,129 @@ MFLBM_HD void zou_he_column(const StepConst<T>& P, int i, int j, int nz, int do_
                 f(i, j, nz, 16, g) = (f(i, j, nz, 17, g) - c6 * tt - tny) * (1 - w) + f(i, j, nz, 16, g) * w;
             }
         }
+    }
+}
+
+// Velocity inlet at k = 1 by bounce-back with the injected momentum of the column's inlet
+// velocity w_in, split between the colours by the injected saturation
+// (src/main_iteration_GPU.cu:1008-1077, inlet_bounce_back_velocity_BC_{before,after}_odd).
+template <typename T, bool AFTER_ODD, class Acc, class Wall, class Phi>
+MFLBM_HD void velocity_inlet_column(const StepConst<T>& P, int i, int j, T w_in, Acc&& f, Wall&& wall, Phi&& ph) {
+    const int w = wall(i, j, 1);
+    T& p0 = ph(i, j, 0);
+    p0 = P.phi_inlet * (1 - w) + p0 * w;
+    const T pv = p0;
+    ph(i, j, -1) = pv;
+    ph(i, j, -2) = pv;
+    ph(i, j, -3) = pv;
+    T t2 = w_in * P.relaxation;
+    const T t1 = t2 * P.sa_inject;  // fluid 1
+    t2 = t2 - t1;                   // fluid 2
+    const T a1 = T(6.0) * (T(1.0) / T(18.0)), a2 = T(6.0) * (T(1.0) / T(36.0));
+    for (int g = 0; g < 2; ++g) {
+        const T tg = g == 0 ? t1 : t2;
+        if (!AFTER_ODD) {
+            f(i, j, 0, 5, g) = (f(i, j, 1, 6, g) + a1 * tg) * (1 - w) + f(i, j, 0, 5, g) * w;
+            f(i - 1, j, 0, 11, g) = (f(i, j, 1, 14, g) + a2 * tg) * (1 - w) + f(i - 1, j, 0, 11, g) * w;
+            f(i + 1, j, 0, 12, g) = (f(i, j, 1, 13, g) + a2 * tg) * (1 - w) + f(i + 1, j, 0, 12, g) * w;
+            f(i, j - 1, 0, 15, g) = (f(i, j, 1, 18, g) + a2 * tg) * (1 - w) + f(i, j - 1, 0, 15, g) * w;
+            f(i, j + 1, 0, 16, g) = (f(i, j, 1, 17, g) + a2 * tg) * (1 - w) + f(i, j + 1, 0, 16, g) * w;
+        } else {
+            f(i, j, 1, 6, g) = (f(i, j, 0, 5, g) + a1 * tg) * (1 - w) + f(i, j, 1, 6, g) * w;
+            f(i, j, 1, 13, g) = (f(i + 1, j, 0, 12, g) + a2 * tg) * (1 - w) + f(i, j, 1, 13, g) * w;
+            f(i, j, 1, 14, g) = (f(i - 1, j, 0, 11, g) + a2 * tg) * (1 - w) + f(i, j, 1, 14, g) * w;
+            f(i, j, 1, 17, g) = (f(i, j + 1, 0, 16, g) + a2 * tg) * (1 - w) + f(i, j, 1, 17, g) * w;
+            f(i, j, 1, 18, g) = (f(i, j - 1, 0, 15, g) + a2 * tg) * (1 - w) + f(i, j, 1, 18, g) * w;
+        }
+    }
+}
+
+// Convective outlet at k = nz: the unknown populations and the ghost phi follow
+// d/dt + u d/dz = 0 with u = P.uin_avg, from the values stored for the column at the last
+// step (cs(e, g) populations, cphi() phi), which are then replaced
+// (src/main_iteration_GPU.cu:1245-1357, outlet_convective_BC_{before,after}_odd).
+template <typename T, bool AFTER_ODD, class Acc, class Wall, class Phi, class CS, class CP>
+MFLBM_HD void convective_outlet_column(const StepConst<T>& P, int i, int j, int nz, Acc&& f, Wall&& wall, Phi&& ph,
+                                       CS&& cs, CP&& cphi) {
+    const T u = P.uin_avg;
+    const T tmp = T(1.0) / (T(1.0) + u);
+    const int w = wall(i, j, nz);
+    T& pg = ph(i, j, nz + 1);
+    pg = ((cphi() + u * ph(i, j, nz)) * tmp) * (1 - w) + pg * w;
+    cphi() = pg;
+    const T pv = pg;
+    ph(i, j, nz + 2) = pv;
+    ph(i, j, nz + 3) = pv;
+    ph(i, j, nz + 4) = pv;
+    for (int g = 0; g < 2; ++g) {
+        if (!AFTER_ODD) {
+            f(i, j, nz + 1, 6, g) = ((cs(6, g) + u * f(i, j, nz, 6, g)) * tmp) * (1 - w) + f(i, j, nz + 1, 6, g) * w;
+            f(i - 1, j, nz + 1, 13, g) =
+                ((cs(13, g) + u * f(i - 1, j, nz, 13, g)) * tmp) * (1 - w) + f(i - 1, j, nz + 1, 13, g) * w;
+            f(i + 1, j, nz + 1, 14, g) =
+                ((cs(14, g) + u * f(i + 1, j, nz, 14, g)) * tmp) * (1 - w) + f(i + 1, j, nz + 1, 14, g) * w;
+            f(i, j - 1, nz + 1, 17, g) =
+                ((cs(17, g) + u * f(i, j - 1, nz, 17, g)) * tmp) * (1 - w) + f(i, j - 1, nz + 1, 17, g) * w;
+            f(i, j + 1, nz + 1, 18, g) =
+                ((cs(18, g) + u * f(i, j + 1, nz, 18, g)) * tmp) * (1 - w) + f(i, j + 1, nz + 1, 18, g) * w;
+        } else {
+            f(i, j, nz, 5, g) = ((cs(6, g) + u * f(i, j, nz - 1, 5, g)) * tmp) * (1 - w) + f(i, j, nz, 5, g) * w;
+            f(i, j, nz, 11, g) = ((cs(14, g) + u * f(i, j, nz - 1, 11, g)) * tmp) * (1 - w) + f(i, j, nz, 11, g) * w;
+            f(i, j, nz, 12, g) = ((cs(13, g) + u * f(i, j, nz - 1, 12, g)) * tmp) * (1 - w) + f(i, j, nz, 12, g) * w;
+            f(i, j, nz, 15, g) = ((cs(18, g) + u * f(i, j, nz - 1, 15, g)) * tmp) * (1 - w) + f(i, j, nz, 15, g) * w;
+            f(i, j, nz, 16, g) = ((cs(17, g) + u * f(i, j, nz - 1, 16, g)) * tmp) * (1 - w) + f(i, j, nz, 16, g) * w;
+        }
+    }
+    for (int g = 0; g < 2; ++g) {
+        if (!AFTER_ODD) {
+            cs(6, g) = f(i, j, nz + 1, 6, g);
+            cs(13, g) = f(i - 1, j, nz + 1, 13, g);
+            cs(14, g) = f(i + 1, j, nz + 1, 14, g);
+            cs(17, g) = f(i, j - 1, nz + 1, 17, g);
+            cs(18, g) = f(i, j + 1, nz + 1, 18, g);
+        } else {
+            cs(6, g) = f(i, j, nz, 5, g);
+            cs(14, g) = f(i, j, nz, 11, g);
+            cs(13, g) = f(i, j, nz, 12, g);
+            cs(18, g) = f(i, j, nz, 15, g);
+            cs(17, g) = f(i, j, nz, 16, g);
+        }
+    }
+}
+
+// Porous plate at k = zp that lets one colour through and reflects the other, `blocked`
+// (0 = fluid 1, 1 = fluid 2) (src/main_iteration_GPU.cu:1744-1882, porous_plate_BC_*).
+template <bool AFTER_ODD, class Acc>
+MFLBM_HD void porous_plate_column(int i, int j, int zp, int blocked, Acc&& f) {
+    const int b = blocked, c = 1 - blocked;
+    if (!AFTER_ODD) {
+        f(i, j, zp, 6, b) = f(i, j, zp - 1, 5, b);
+        f(i - 1, j, zp, 13, b) = f(i, j, zp - 1, 12, b);
+        f(i + 1, j, zp, 14, b) = f(i, j, zp - 1, 11, b);
+        f(i, j - 1, zp, 17, b) = f(i, j, zp - 1, 16, b);
+        f(i, j + 1, zp, 18, b) = f(i, j, zp - 1, 15, b);
+        f(i, j, zp, 5, b) = f(i, j, zp + 1, 6, b);
+        f(i + 1, j, zp, 12, b) = f(i, j, zp + 1, 13, b);
+        f(i - 1, j, zp, 11, b) = f(i, j, zp + 1, 14, b);
+        f(i, j + 1, zp, 16, b) = f(i, j, zp + 1, 17, b);
+        f(i, j - 1, zp, 15, b) = f(i, j, zp + 1, 18, b);
+        const int up[5] = {6, 13, 14, 17, 18}, dn[5] = {5, 12, 11, 16, 15};
+        for (int m = 0; m < 5; ++m) f(i, j, zp, up[m], c) = f(i, j, zp + 1, up[m], c);
+        for (int m = 0; m < 5; ++m) f(i, j, zp, dn[m], c) = f(i, j, zp - 1, dn[m], c);
+    } else {
+        f(i, j, zp - 1, 5, b) = f(i, j, zp, 6, b);
+        f(i, j, zp - 1, 12, b) = f(i - 1, j, zp, 13, b);
+        f(i, j, zp - 1, 11, b) = f(i + 1, j, zp, 14, b);
+        f(i, j, zp - 1, 16, b) = f(i, j - 1, zp, 17, b);
+        f(i, j, zp - 1, 15, b) = f(i, j + 1, zp, 18, b);
+        f(i, j, zp + 1, 6, b) = f(i, j, zp, 5, b);
+        f(i, j, zp + 1, 13, b) = f(i + 1, j, zp, 12, b);
+        f(i, j, zp + 1, 14, b) = f(i - 1, j, zp, 11, b);
+        f(i, j, zp + 1, 17, b) = f(i, j + 1, zp, 16, b);
+        f(i, j, zp + 1, 18, b) = f(i, j - 1, zp, 15, b);
+        const int up[5] = {6, 13, 14, 17, 18}, dn[5] = {5, 12, 11, 16, 15};
+        for (int m = 0; m < 5; ++m) f(i, j, zp - 1, dn[m], c) = f(i, j, zp, dn[m], c);
+        for (int m = 0; m < 5; ++m) f(i, j, zp + 1, up[m], c) = f(i, j, zp, up[m], c);
     }
 }
 

@@ -20,8 +20,11 @@
 //   * the device state is copied to the host only for the outputs that need it;
 //   * precision is a run-time option (--precision fp64|fp32) rather than a build switch;
 //     a checkpoint of either precision can be resumed by either.
-// Velocity inlet, convective outlet, porous plate, geometry modification, precomputed
-// boundary files and the inlet phase switch are not built; they are rejected up front.
+//   * the convective-outlet tail of the checkpoint is written when that outlet is active
+//     (outlet_BC = 1 on an open z axis); the reference also writes it, unused, when
+//     outlet_BC = 1 on a closed or periodic z axis.  Files with or without it load.
+// Geometry modification, precomputed boundary files and the inlet phase switch are not
+// built; they are rejected up front.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -88,10 +91,14 @@ struct Control {
     int wall[6];  // x min/max, y min/max, z min/max
     int iper, jper, kper, inlet_bc, outlet_bc;
     int max_step, ntime_visual, ntime_animation, ntime_monitor, monitor_profile_ratio;
-    int ntime_clock_sum, ntime_display, rho_in_new, rho_out_bc;
+    int ntime_clock_sum, ntime_display, rho_in_new, rho_out_bc, z_porous_plate;
     long long nx, ny, nz;
     double convergence, nu1, nu2, gamma, theta, beta, sa_inject, interface_z0, body_force;
     double checkpoint_timer, checkpoint2_timer, duration_timer;
+    double capillary_number, target_inject_pore_volume, d_vol_animation, d_vol_detail, d_vol_monitor;
+    bool open_z() const { return kper == 0 && wall[4] == 0 && wall[5] == 0; }
+    bool velocity_inlet() const { return open_z() && inlet_bc == 1; }
+    bool convective_outlet() const { return open_z() && outlet_bc == 1; }
 };
 
 Control read_control(const std::string& path) {
@@ -110,6 +117,7 @@ Control read_control(const std::string& path) {
     if (c.dims_type_size != 4 && c.dims_type_size != 8) throw Fatal("Incorrect geometry_dims_type_size!");
     c.geometry_preprocess = f.i32("geometry_preprocess_cmd");
     c.porous_plate = f.i32("porous_plate_cmd");
+    c.z_porous_plate = f.i32("Z_porous_plate");
     c.change_inlet_phase = f.i32("change_inlet_fluid_phase_cmd");
     c.nx = f.i64("nxGlobal");
     c.ny = f.i64("nyGlobal");
@@ -147,6 +155,11 @@ Control read_control(const std::string& path) {
     c.duration_timer = f.real("simulation_duration_timer");
     c.rho_in_new = f.i32("rho_in_new");
     c.rho_out_bc = f.i32("rho_out_BC");
+    c.capillary_number = f.real("capillary_number");
+    c.target_inject_pore_volume = f.real("target_inject_pore_volume");
+    c.d_vol_animation = f.real("d_vol_animation");
+    c.d_vol_detail = f.real("d_vol_detail");
+    c.d_vol_monitor = f.real("d_vol_monitor");
     return c;
 }
 
@@ -163,10 +176,7 @@ void check_control(const Control& c) {
     if (c.kper == 1 && (c.wall[5] == 1 || c.wall[4] == 1))
         err.push_back("non-slip BC applied at z = zmin or z = zmax while z direction periodic BC enabled");
     if (c.outlet_bc == 1 && c.inlet_bc == 2) err.push_back("Inlet pressure + outlet convective BC is not supported");
-    const bool open_z = c.kper == 0 && c.wall[4] == 0 && c.wall[5] == 0;
-    if (open_z && (c.inlet_bc == 1 || c.outlet_bc == 1))
-        err.push_back("velocity inlet / convective outlet boundary conditions are not built");
-    if (c.porous_plate != 0) err.push_back("porous plate (porous_plate_cmd != 0) is not built");
+    if (c.porous_plate < 0 || c.porous_plate > 2) err.push_back("porous_plate_cmd must be 0, 1 or 2");
     if (c.modify_geometry != 0) err.push_back("modify_geometry_cmd = 1 (hard-coded geometry) is not built");
     if (c.geometry_preprocess != 0)
         err.push_back("geometry_preprocess_cmd = 1 is not built: boundary info is computed on the device");
@@ -208,7 +218,14 @@ mflbm_params derive(const Control& c, int precision) {
     p.force_z = c.body_force;
     p.rho_out = 1.0;
     p.rho_in = 0.0;
-    if (c.kper == 0 && c.wall[4] == 0 && c.wall[5] == 0 && c.inlet_bc == 2) {
+    p.porous_plate = c.porous_plate;
+    p.z_porous_plate = c.z_porous_plate;
+    p.uin_avg = 0.0;
+    if (c.velocity_inlet()) {  // src/Init_multiphase.cpp:195-199, 262-263
+        p.force_z = 0.0;
+        p.uin_avg = c.capillary_number * c.gamma / c.nu1;
+    }
+    if (c.open_z() && c.inlet_bc == 2) {
         p.force_z = 0.0;
         const double p_gradient = -c.body_force / 3.0;
         if (c.rho_out_bc) p.rho_out = 1.0 - p_gradient * (double)c.nz;
@@ -336,8 +353,10 @@ private:
     std::vector<double> download_real(int field, int64_t n) const;
     void upload_real(int field, const std::vector<double>& v);
     void geometry_info();
+    void injected_volume_timers();
     void write_info() const;
-    void load_checkpoint(double* force_z, double* rho_in, std::vector<double>* pdf, std::vector<double>* phi);
+    void load_checkpoint(double* force_z, double* rho_in, std::vector<double>* pdf, std::vector<double>* phi,
+                         std::vector<double>* conv);
     void save_checkpoint(int where);
     void monitor();
     void monitor_phase_field();
@@ -352,7 +371,7 @@ private:
         return f;
     }
 
-    const Control& c_;
+    Control c_;  // a copy: the velocity inlet rescales the timers by the injected volume
     const int prec_;
     mflbm_handle* h_ = nullptr;
     mflbm_params p_{};
@@ -410,6 +429,48 @@ void Simulation::geometry_info() {
     a_xy_ = la_x * la_y;
 }
 
+// Timers of a velocity-inlet run set by the injected volume, in pore volumes, over the inlet
+// flow rate uin_avg * A_xy, rounded up to even steps (src/Init_multiphase.cpp:218-248, 286-292);
+// computed in the run's precision as the reference computes them in T_P.
+void Simulation::injected_volume_timers() {
+    if (!c_.velocity_inlet()) return;
+    auto steps = [&](double vol) -> int {
+        int t;
+        if (prec_ == MFLBM_FP64) {
+            const double flowrate = p_.uin_avg * a_xy_;
+            t = (int)((double)(vol * (double)pore_sum_) / flowrate);
+        } else {
+            const float la_x = (float)(nx_ - 1) - 0.5f - 0.5f, la_y = (float)(ny_ - 1) - 0.5f - 0.5f;
+            const float uin = (float)c_.capillary_number * (float)c_.gamma / (float)c_.nu1;
+            const float flowrate = uin * (la_x * la_y);
+            t = (int)((float)((float)vol * (float)pore_sum_) / flowrate);
+        }
+        return t % 2 == 1 ? t + 1 : t;
+    };
+    std::cout << "Inlet average velocity = " << p_.uin_avg << std::endl;
+    std::cout << "Inlet flowrate = " << p_.uin_avg * a_xy_ << std::endl;
+    if (c_.target_inject_pore_volume > 0) {
+        c_.max_step = steps(c_.target_inject_pore_volume);
+        std::cout << "Maximum time step is modified based on target inject volume! ntime_max = " << c_.max_step
+                  << std::endl;
+    }
+    if (c_.d_vol_animation > 0) {
+        c_.ntime_animation = steps(c_.d_vol_animation);
+        std::cout << "Animation VTK files timer is modified based on injected volume: " << c_.ntime_animation << std::endl;
+    }
+    if (c_.d_vol_detail > 0) {
+        c_.ntime_visual = steps(c_.d_vol_detail);
+        std::cout << "Full VTK files timer is modified based on injected volume: " << c_.ntime_visual << std::endl;
+    }
+    if (c_.d_vol_monitor > 0) {
+        c_.ntime_monitor = steps(c_.d_vol_monitor);
+        ntime_monitor_profile_ = c_.monitor_profile_ratio * c_.ntime_monitor;
+        std::cout << "Monitor timer is modified based on injected volume: " << c_.ntime_monitor << std::endl;
+    }
+    if (c_.ntime_animation <= 0 || c_.ntime_visual <= 0 || c_.ntime_monitor <= 0)
+        throw Fatal("Error: an injected-volume timer is below one step (raise d_vol_* or lower the flow rate)!");
+}
+
 void Simulation::write_info() const {
     std::ofstream f("results/out1.output/info.txt");
     if (!f.good()) throw Fatal("Couldn't open the file results/out1.output/info.txt !");
@@ -430,21 +491,27 @@ void Simulation::write_info() const {
     std::cout << "Full domain porosity = " << porosity_full << std::endl;
 }
 
-// Checkpoint: int32 (ntime + 1) | T force_z | T rho_in | pdf (38 s1 T) | phi (s4 T), native
-// endian, T the run's precision (src/IO_multiphase.cpp:252-287).  Either precision loads.
+// Checkpoint: int32 (ntime + 1) | T force_z | T rho_in | pdf (38 s1 T) | phi (s4 T)
+// [| f_convec, g_convec (19 X1 Y1 T each) | phi_convec (X1 Y1 T) with the convective outlet],
+// native endian, T the run's precision (src/IO_multiphase.cpp:252-287).  Either precision loads.
 void Simulation::load_checkpoint(double* force_z, double* rho_in, std::vector<double>* pdf,
-                                 std::vector<double>* phi) {
+                                 std::vector<double>* phi, std::vector<double>* conv) {
     const std::string path = "results/out2.checkpoint/id0000";
     FILE* fp = std::fopen(path.c_str(), "rb");
     if (!fp) throw Fatal("Checkpoint data not found! Exiting program!");
     std::fseek(fp, 0, SEEK_END);
     const long long bytes = std::ftell(fp);
     std::fseek(fp, 0, SEEK_SET);
-    const long long body = 2 + 38 * s1_ + s4_;
-    size_t es;
-    if (bytes == 4 + (long long)sizeof(double) * body) es = sizeof(double);
-    else if (bytes == 4 + (long long)sizeof(float) * body) es = sizeof(float);
-    else { std::fclose(fp); throw Fatal("Checkpoint size does not match the domain"); }
+    const long long tail = 39 * (long long)(nx_ + 2) * (ny_ + 2), base = 2 + 38 * (long long)s1_ + s4_;
+    long long body = 0;
+    size_t es = 0;
+    for (long long b : {base, base + tail})
+        for (size_t e : {sizeof(double), sizeof(float)})
+            if (bytes == 4 + (long long)e * b) { body = b; es = e; }
+    if (!es) { std::fclose(fp); throw Fatal("Checkpoint size does not match the domain"); }
+    const bool has_tail = body > base;
+    if (c_.convective_outlet() && !has_tail)
+        { std::fclose(fp); throw Fatal("Could not load from data check_point_file file! (no convective-outlet state)"); }
     int32_t nt;
     bool ok = std::fread(&nt, 4, 1, fp) == 1;
     std::vector<double> all((size_t)body);
@@ -460,7 +527,9 @@ void Simulation::load_checkpoint(double* force_z, double* rho_in, std::vector<do
     *force_z = all[0];
     *rho_in = all[1];
     pdf->assign(all.begin() + 2, all.begin() + 2 + 38 * s1_);
-    phi->assign(all.begin() + 2 + 38 * s1_, all.end());
+    phi->assign(all.begin() + 2 + 38 * s1_, all.begin() + 2 + 38 * s1_ + s4_);
+    conv->clear();
+    if (has_tail && c_.convective_outlet()) conv->assign(all.begin() + 2 + 38 * s1_ + s4_, all.end());
     ntime0_ = nt;  // the first step not yet taken
 }
 
@@ -484,6 +553,12 @@ void Simulation::save_checkpoint(int where) {
     }
     put(pdf.data(), (size_t)(38 * s1_));
     put(phi.data(), (size_t)s4_);
+    if (c_.convective_outlet()) {
+        const size_t n = (size_t)(39 * (nx_ + 2) * (ny_ + 2));
+        std::vector<char> conv(n * es_);
+        check(mflbm_download(h_, MFLBM_CONVEC, conv.data(), conv.size()), "download convective outlet state");
+        put(conv.data(), n);
+    }
     std::fclose(fp);
     if (!ok) throw Fatal("Could not write the checkpoint file " + path);
     if (where == 0) {
@@ -726,11 +801,11 @@ int Simulation::run(const std::string& job_status) {
         make_dir(d);
     const std::vector<int8_t> geo = read_geometry(c_);
     p_ = derive(c_, prec_);
-    std::vector<double> ck_pdf, ck_phi;
+    std::vector<double> ck_pdf, ck_phi, ck_conv;
     if (job_status == "continue_simulation") {
         std::cout << "loading checkpoint data ... ";
         double fz, ri;
-        load_checkpoint(&fz, &ri, &ck_pdf, &ck_phi);
+        load_checkpoint(&fz, &ri, &ck_pdf, &ck_phi, &ck_conv);
         p_.force_z = fz;
         p_.rho_in = ri;
         std::cout << "Complete" << std::endl;
@@ -745,6 +820,7 @@ int Simulation::run(const std::string& job_status) {
     std::cout << "Fluid 2 relaxation time = " << 3.0 * c_.nu2 + 0.5 << std::endl;
     create(p_, geo);
     geometry_info();
+    injected_volume_timers();
     write_info();
     if (job_status == "new_simulation") {
         ntime0_ = 1;
@@ -753,6 +829,7 @@ int Simulation::run(const std::string& job_status) {
     } else {
         upload_real(MFLBM_PDF, ck_pdf);
         upload_real(MFLBM_PHI, ck_phi);
+        if (!ck_conv.empty()) upload_real(MFLBM_CONVEC, ck_conv);
         monitor_previous_ = -1.0;  // no convergence test on the first monitor after a restart
         monitor_current_ = -1.0;
     }

@@ -157,6 +157,12 @@ def derive(ctrl: dict) -> dict:
         "rho_in": rho_in,
         "rho_out": rho_out,
         "relaxation": 1.0,
+        "porous_plate": int(ctrl.get("porous_plate_cmd", 0)),
+        "z_porous_plate": int(ctrl.get("Z_porous_plate", 0)),
+        # mean inlet velocity, set only for a velocity inlet on an open z axis (src/Init_multiphase.cpp:195-199,
+        # 262-263); the convective outlet advects with it, so with another inlet it advects with 0
+        "uin_avg": (float(ctrl.get("capillary_number", 0.0)) * float(ctrl["surface_tension"]) / la_nu1
+                    if open_z and int(ctrl["inlet_BC"]) == 1 else 0.0),
         # not step constants, kept for initialisation / monitors
         "la_nu1": la_nu1, "la_nu2": la_nu2,
         "initial_fluid_distribution_option": int(ctrl["initial_fluid_distribution_option"]),

@@ -18,7 +18,7 @@ HEADER = os.path.join(REPO, "include", "mflbm.h")
 
 FP64, FP32 = 0, 1
 FIELDS = {"pdf": 0, "phi": 1, "cn_x": 2, "cn_y": 3, "cn_z": 4, "c_norm": 5, "curv": 6,
-          "s_nx": 7, "s_ny": 8, "s_nz": 9, "walls": 10, "walls_type": 11}
+          "s_nx": 7, "s_ny": 8, "s_nz": 9, "walls": 10, "walls_type": 11, "w_in": 12, "convec": 13}
 HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv": 1,
         "s_nx": 4, "s_ny": 4, "s_nz": 4, "walls": 2, "walls_type": 4}
 
@@ -42,6 +42,7 @@ class Params(C.Structure):
         ("lbm_beta", C.c_double), ("phi_inlet", C.c_double), ("sa_inject", C.c_double), ("cos_theta", C.c_double),
         ("rho_in", C.c_double), ("rho_out", C.c_double), ("relaxation", C.c_double),
         ("precision", C.c_int32), ("flags", C.c_int32),
+        ("porous_plate", C.c_int32), ("z_porous_plate", C.c_int32), ("uin_avg", C.c_double),
     ]
 
 
@@ -216,6 +217,10 @@ class Solver:
         self._check(load().mflbm_download(self.h, f, out.ctypes.data_as(C.c_void_p), out.nbytes))
         if name == "pdf":
             return out.reshape((2, 19) + self.shape(1))
+        if name == "w_in":
+            return out.reshape(self.shape(1)[1:])
+        if name == "convec":
+            return out.reshape((39,) + self.shape(1)[1:])
         return out.reshape(self.shape(HALO[name]))
 
     def upload(self, name, arr):

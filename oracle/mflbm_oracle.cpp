@@ -55,6 +55,9 @@ typedef struct oracle_params {
     double rho_in;
     double rho_out;
     double relaxation;  // 1 in the reference (src/main.cpp:57)
+    int32_t porous_plate;    // porous_plate_cmd: 0 none, 1 blocks fluid 1, 2 blocks fluid 2
+    int32_t z_porous_plate;  // Z_porous_plate
+    double uin_avg;          // capillary_number * surface_tension / nu1 (src/Init_multiphase.cpp:262)
 } oracle_params;
 
 typedef struct oracle_monitor {
@@ -78,6 +81,7 @@ struct Consts {
     double w[19];
     double w1, w2, rk2;
     double r0, r1, r2;  // rest weights of the collision's equilibrium
+    double pi;
     double iso4[2];
     double iso8[7];
     double c19, c2394, c252, c72;   // MRT back-transform coefficients
@@ -91,6 +95,7 @@ struct Consts {
         w2 = 1.0 / 36.0;
         // rest state of the collision (e = -11 rho, e2 = e2c1 rho = 0): no reference counterpart
         r0 = 4.0 / 21.0; r1 = 13.0 / 126.0; r2 = 1.0 / 63.0;
+        pi = 3.14159265358979323846;  // includes/Module.h:9
         rk2 = 1.0 / std::sqrt(2.0) / 36.0;           // includes/Fluid_multiphase.h:32
         iso4[0] = 1.0 / 6.0;  iso4[1] = 1.0 / 12.0;  // includes/Fluid_multiphase.h:34
         iso8[0] = 4.0 / 45.0; iso8[1] = 1.0 / 21.0; iso8[2] = 2.0 / 105.0; iso8[3] = 5.0 / 504.0;
@@ -168,6 +173,8 @@ struct OracleSim {
     std::vector<double> cn_x, cn_y, cn_z, c_norm;  // s2
     std::vector<double> curv;         // s1
     std::vector<int32_t> pore_profile_z;
+    std::vector<double> w_in;         // X1 * Y1: velocity-inlet profile (plane k = 0 of W_in)
+    std::vector<double> conv;         // 39 * X1 * Y1: f_convec, g_convec ([e][y][x]), phi_convec
     int64_t n_solid_boundary = 0, n_fluid_boundary = 0;
 
     // index helpers: i,j,k are 1-based interior coordinates as in includes/Idx_cpu.h:49-66
@@ -178,6 +185,9 @@ struct OracleSim {
     inline double& F(int64_t i, int64_t j, int64_t k, int e, int g) { return pdf[f1(i, j, k, e, g)]; }
 
     bool open_z() const { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
+    inline int64_t col(int64_t i, int64_t j) const { return i + X1 * j; }
+    inline double& CF(int64_t i, int64_t j, int e, int g) { return conv[col(i, j) + X1 * Y1 * (e + 19 * g)]; }
+    inline double& CPHI(int64_t i, int64_t j) { return conv[col(i, j) + X1 * Y1 * 38]; }
 };
 
 namespace {
@@ -910,6 +920,156 @@ void gradient_chain(OracleSim& S) {
 }
 
 // =====================================================================================
+//  Velocity inlet (src/main_iteration_GPU.cu:1008-1077): bounce-back plus the momentum of
+//  the inlet profile, per colour by the injected saturation.  Profile and mean velocity:
+//  src/Init_multiphase.cpp:258-281, src/Misc.cpp:387-419 (1000-term duct series).
+// =====================================================================================
+void velocity_profile(OracleSim& S) {
+    const int64_t nx = S.nx, ny = S.ny;
+    S.w_in.assign(S.X1 * S.Y1, 0.0);
+    const double U = S.p.uin_avg;
+    for (int64_t y = 1; y <= ny; ++y)
+        for (int64_t x = 1; x <= nx; ++x)
+            if (x > 1 && x < nx && y > 1 && y < ny) S.w_in[S.col(x, y)] = U;
+    const double la_x = double(nx - 1) - 0.5 - 0.5, la_y = double(ny - 1) - 0.5 - 0.5;
+    const double a = 0.5 * la_x, b = 0.5 * la_y;
+    double sum_t = 0.0;
+    for (int64_t n = 1; n <= 1000; n += 2) sum_t += std::tanh(0.5 * double(n) * K.pi * b / a) / std::pow(double(n), 5);
+    double c = 1.0 - 192.0 / std::pow(K.pi, 5) * (a / b) * sum_t;
+    c = -3.0 * U / (c * std::pow(a, 2));
+    for (int64_t y = 1; y <= ny; ++y)
+        for (int64_t x = 1; x <= nx; ++x) {
+            if (!(x > 1 && x < nx && y > 1 && y < ny)) continue;
+            const double xx = x - 1.5 - a, yy = y - 1.5 - b;
+            double acc = 0.0;
+            for (int64_t n = 1; n <= 1000; n += 2) {
+                acc += std::pow(-1.0, 0.5 * double(n - 1)) * std::cos(0.5 * double(n) * K.pi * xx / a) / std::pow(double(n), 3) *
+                       (1.0 - (std::exp(0.5 * double(n) * K.pi * (yy - b) / a) + std::exp(0.5 * double(n) * K.pi * (-yy - b) / a)) /
+                                  (1.0 + std::exp(0.5 * double(n) * K.pi * (-b - b) / a)));
+            }
+            S.w_in[S.col(x, y)] = acc * (-16.0 * c * std::pow(a, 2) * std::pow(K.pi, -3));
+        }
+}
+
+void inlet_velocity(OracleSim& S, bool after_odd) {
+    const double a1 = 6.0 * K.w1, a2 = 6.0 * K.w2;
+    for (int64_t j = 1; j <= S.ny; ++j)
+        for (int64_t i = 1; i <= S.nx; ++i) {
+            const int w = S.walls[S.s2(i, j, 1)];
+            double& p0 = S.phi[S.s4(i, j, 0)];
+            p0 = S.p.phi_inlet * (1 - w) + p0 * w;
+            for (int k = -1; k >= -3; --k) S.phi[S.s4(i, j, k)] = p0;
+            double inj[2];
+            inj[1] = S.w_in[S.col(i, j)] * S.p.relaxation;
+            inj[0] = inj[1] * S.p.sa_inject;
+            inj[1] = inj[1] - inj[0];
+            for (int g = 0; g < 2; ++g) {
+                auto f = [&](int64_t a, int64_t b, int64_t c, int e) -> double& { return S.F(a, b, c, e, g); };
+                const double t = inj[g];
+                if (!after_odd) {
+                    f(i, j, 0, 5) = (f(i, j, 1, 6) + a1 * t) * (1 - w) + f(i, j, 0, 5) * w;
+                    f(i - 1, j, 0, 11) = (f(i, j, 1, 14) + a2 * t) * (1 - w) + f(i - 1, j, 0, 11) * w;
+                    f(i + 1, j, 0, 12) = (f(i, j, 1, 13) + a2 * t) * (1 - w) + f(i + 1, j, 0, 12) * w;
+                    f(i, j - 1, 0, 15) = (f(i, j, 1, 18) + a2 * t) * (1 - w) + f(i, j - 1, 0, 15) * w;
+                    f(i, j + 1, 0, 16) = (f(i, j, 1, 17) + a2 * t) * (1 - w) + f(i, j + 1, 0, 16) * w;
+                } else {
+                    f(i, j, 1, 6) = (f(i, j, 0, 5) + a1 * t) * (1 - w) + f(i, j, 1, 6) * w;
+                    f(i, j, 1, 13) = (f(i + 1, j, 0, 12) + a2 * t) * (1 - w) + f(i, j, 1, 13) * w;
+                    f(i, j, 1, 14) = (f(i - 1, j, 0, 11) + a2 * t) * (1 - w) + f(i, j, 1, 14) * w;
+                    f(i, j, 1, 17) = (f(i, j + 1, 0, 16) + a2 * t) * (1 - w) + f(i, j, 1, 17) * w;
+                    f(i, j, 1, 18) = (f(i, j - 1, 0, 15) + a2 * t) * (1 - w) + f(i, j, 1, 18) * w;
+                }
+            }
+        }
+}
+
+// =====================================================================================
+//  Convective outlet (src/main_iteration_GPU.cu:1245-1357): unknown populations and the
+//  ghost phi advected out with the mean inlet velocity from the values of the last step.
+// =====================================================================================
+void outlet_convective(OracleSim& S, bool after_odd) {
+    const int64_t nz = S.nz;
+    const double u = S.p.uin_avg, r = 1.0 / (1.0 + u);
+    for (int64_t j = 1; j <= S.ny; ++j)
+        for (int64_t i = 1; i <= S.nx; ++i) {
+            const int w = S.walls[S.s2(i, j, nz)];
+            double& pg = S.phi[S.s4(i, j, nz + 1)];
+            pg = ((S.CPHI(i, j) + u * S.phi[S.s4(i, j, nz)]) * r) * (1 - w) + pg * w;
+            S.CPHI(i, j) = pg;
+            for (int k = 2; k <= 4; ++k) S.phi[S.s4(i, j, nz + k)] = pg;
+            // (stored slot, target cell, slot read one plane inside)
+            struct Link { int e; int64_t a, b, c; int er; };
+            const Link before[5] = {{6, i, j, nz + 1, 6}, {13, i - 1, j, nz + 1, 13}, {14, i + 1, j, nz + 1, 14},
+                                    {17, i, j - 1, nz + 1, 17}, {18, i, j + 1, nz + 1, 18}};
+            const Link after[5] = {{6, i, j, nz, 5}, {14, i, j, nz, 11}, {13, i, j, nz, 12},
+                                   {18, i, j, nz, 15}, {17, i, j, nz, 16}};
+            const Link* L = after_odd ? after : before;
+            for (int g = 0; g < 2; ++g)
+                for (int m = 0; m < 5; ++m) {
+                    const Link& l = L[m];
+                    double& t = S.F(l.a, l.b, l.c, l.er, g);
+                    const double inner = S.F(l.a, l.b, l.c - 1, l.er, g);
+                    t = ((S.CF(i, j, l.e, g) + u * inner) * r) * (1 - w) + t * w;
+                }
+            for (int g = 0; g < 2; ++g)
+                for (int m = 0; m < 5; ++m) {
+                    const Link& l = L[m];
+                    S.CF(i, j, l.e, g) = S.F(l.a, l.b, l.c, l.er, g);
+                }
+        }
+}
+
+// state of a new run: the outlet plane of the initial populations and phi
+// (src/Init_multiphase.cpp:445-494)
+void convective_init(OracleSim& S) {
+    S.conv.assign(39 * S.X1 * S.Y1, 0.0);
+    for (int64_t j = 0; j <= S.ny + 1; ++j)
+        for (int64_t i = 0; i <= S.nx + 1; ++i) {
+            for (int g = 0; g < 2; ++g)
+                for (int e = 0; e < 19; ++e) S.CF(i, j, e, g) = S.F(i, j, S.nz, e, g);
+            S.CPHI(i, j) = S.phi[S.s4(i, j, S.nz)];
+        }
+}
+
+// =====================================================================================
+//  Porous plate at k = zp (src/main_iteration_GPU.cu:1744-1882): the blocked colour is
+//  reflected on both faces of the plate, the other passes.
+// =====================================================================================
+void porous_plate(OracleSim& S, bool after_odd) {
+    const int64_t zp = S.p.z_porous_plate;
+    if (S.p.porous_plate == 0 || zp < 1 || zp > S.nz) return;
+    const int b = S.p.porous_plate == 1 ? 0 : 1, c = 1 - b;
+    const int up[5] = {6, 13, 14, 17, 18}, dn[5] = {5, 12, 11, 16, 15};
+    // reflection pairs: (slot of the plate-plane cell, its lateral offset, slot read)
+    const int rx[5] = {0, -1, 1, 0, 0}, ry[5] = {0, 0, 0, -1, 1};
+    for (int64_t j = 1; j <= S.ny; ++j)
+        for (int64_t i = 1; i <= S.nx; ++i) {
+            if (!after_odd) {
+                // f(i+dx, j+dy, zp, up) <- f(i, j, zp-1, dn); f(i-dx, j-dy, zp, dn) <- f(i, j, zp+1, up)
+                for (int m = 0; m < 5; ++m) S.F(i + rx[m], j + ry[m], zp, up[m], b) = S.F(i, j, zp - 1, dn[m], b);
+                for (int m = 0; m < 5; ++m) S.F(i - rx[m], j - ry[m], zp, dn[m], b) = S.F(i, j, zp + 1, up[m], b);
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp, up[m], c) = S.F(i, j, zp + 1, up[m], c);
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp, dn[m], c) = S.F(i, j, zp - 1, dn[m], c);
+            } else {
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp - 1, dn[m], b) = S.F(i + rx[m], j + ry[m], zp, up[m], b);
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp + 1, up[m], b) = S.F(i - rx[m], j - ry[m], zp, dn[m], b);
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp - 1, dn[m], c) = S.F(i, j, zp, dn[m], c);
+                for (int m = 0; m < 5; ++m) S.F(i, j, zp + 1, up[m], c) = S.F(i, j, zp, up[m], c);
+            }
+        }
+}
+
+void open_boundaries(OracleSim& S, bool odd) {
+    if (S.open_z()) {
+        if (S.p.inlet_bc == 1) inlet_velocity(S, odd);
+        else if (S.p.inlet_bc == 2) inlet_zou_he(S, odd);
+        if (S.p.outlet_bc == 1) outlet_convective(S, odd);
+        else if (S.p.outlet_bc == 2) outlet_zou_he(S, odd);
+    }
+    porous_plate(S, odd);
+}
+
+// =====================================================================================
 //  One time step (src/main_iteration_GPU.cu:1890-2055)
 // =====================================================================================
 void step(OracleSim& S, int ntime) {
@@ -918,10 +1078,7 @@ void step(OracleSim& S, int ntime) {
     if (S.p.kper) { periodic_z(S, odd); periodic_phi(S, true, false); }
     if (S.p.jper) { periodic_y(S, odd); periodic_phi(S, false, true); }
     if (S.p.jper && S.p.kper) { periodic_zy(S, odd); periodic_phi(S, true, true, true); }
-    if (S.open_z()) {
-        if (S.p.inlet_bc == 2) inlet_zou_he(S, odd);
-        if (S.p.outlet_bc == 2) outlet_zou_he(S, odd);
-    }
+    open_boundaries(S, odd);
     gradient_chain(S);
 }
 
@@ -934,7 +1091,6 @@ extern "C" {
 
 void* oracle_create(const oracle_params* p, const int8_t* geometry) {
     if (!p || !geometry) return nullptr;
-    if (p->inlet_bc == 1 || p->outlet_bc == 1) return nullptr;  // velocity / convective BCs not restated
     OracleSim* S = new OracleSim();
     S->p = *p;
     S->nx = p->nx; S->ny = p->ny; S->nz = p->nz;
@@ -950,6 +1106,8 @@ void* oracle_create(const oracle_params* p, const int8_t* geometry) {
     S->cn_x.assign(S->S2, 0.0); S->cn_y.assign(S->S2, 0.0); S->cn_z.assign(S->S2, 0.0); S->c_norm.assign(S->S2, 0.0);
     S->curv.assign(S->S1, 0.0);
     preprocess_geometry(*S, geometry);
+    if (S->open_z() && p->inlet_bc == 1) velocity_profile(*S);
+    if (S->open_z() && p->outlet_bc == 1) S->conv.assign(39 * S->X1 * S->Y1, 0.0);
     return S;
 }
 
@@ -962,11 +1120,15 @@ int oracle_init_phi(void* h, int option, double interface_z0) {
 }
 
 void oracle_init_pdf(void* h, const double* rho, const double* u, const double* v, const double* w) {
-    init_pdf(*static_cast<OracleSim*>(h), rho, u, v, w);
+    OracleSim& S = *static_cast<OracleSim*>(h);
+    init_pdf(S, rho, u, v, w);
+    if (S.open_z() && S.p.outlet_bc == 1) convective_init(S);
 }
 
 void oracle_init_pdf_relaxed(void* h, const double* rho, const double* u, const double* v, const double* w) {
-    init_pdf(*static_cast<OracleSim*>(h), rho, u, v, w, true);
+    OracleSim& S = *static_cast<OracleSim*>(h);
+    init_pdf(S, rho, u, v, w, true);
+    if (S.open_z() && S.p.outlet_bc == 1) convective_init(S);
 }
 
 void oracle_color_gradient(void* h) { gradient_chain(*static_cast<OracleSim*>(h)); }
@@ -985,10 +1147,7 @@ void oracle_boundaries(void* h, int odd) {
     if (S.p.kper) { periodic_z(S, odd); periodic_phi(S, true, false); }
     if (S.p.jper) { periodic_y(S, odd); periodic_phi(S, false, true); }
     if (S.p.jper && S.p.kper) { periodic_zy(S, odd); periodic_phi(S, true, true, true); }
-    if (S.open_z()) {
-        if (S.p.inlet_bc == 2) inlet_zou_he(S, odd);
-        if (S.p.outlet_bc == 2) outlet_zou_he(S, odd);
-    }
+    open_boundaries(S, odd);
 }
 
 // Collision of a single node with explicit inputs (for the MRT pinning test).
@@ -997,7 +1156,8 @@ double oracle_collide_node(const oracle_params* p, const double* g1, const doubl
     return collide_node(*p, g1, g2, cnx, cny, cnz, curv, cnorm, out1, out2);
 }
 
-// Array access: 0 pdf, 1 phi, 2 cn_x, 3 cn_y, 4 cn_z, 5 c_norm, 6 curv, 7 s_nx, 8 s_ny, 9 s_nz
+// Array access: 0 pdf, 1 phi, 2 cn_x, 3 cn_y, 4 cn_z, 5 c_norm, 6 curv, 7 s_nx, 8 s_ny, 9 s_nz,
+// 10 w_in (X1 * Y1, velocity inlet), 11 conv (39 * X1 * Y1, convective outlet)
 void* oracle_array(void* h, int which, int64_t* count) {
     OracleSim& S = *static_cast<OracleSim*>(h);
     std::vector<double>* v = nullptr;
@@ -1012,6 +1172,8 @@ void* oracle_array(void* h, int which, int64_t* count) {
         case 7: v = &S.s_nx; break;
         case 8: v = &S.s_ny; break;
         case 9: v = &S.s_nz; break;
+        case 10: v = &S.w_in; break;
+        case 11: v = &S.conv; break;
         default: return nullptr;
     }
     if (count) *count = (int64_t)v->size();

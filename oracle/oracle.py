@@ -26,6 +26,7 @@ class OracleParams(C.Structure):
         ("lbm_gamma", C.c_double), ("force_z", C.c_double), ("la_nui1", C.c_double), ("la_nui2", C.c_double),
         ("lbm_beta", C.c_double), ("phi_inlet", C.c_double), ("sa_inject", C.c_double), ("cos_theta", C.c_double),
         ("rho_in", C.c_double), ("rho_out", C.c_double), ("relaxation", C.c_double),
+        ("porous_plate", C.c_int32), ("z_porous_plate", C.c_int32), ("uin_avg", C.c_double),
     ]
 
 
@@ -87,7 +88,8 @@ def _dp(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-ARRAYS = {"pdf": 0, "phi": 1, "cn_x": 2, "cn_y": 3, "cn_z": 4, "c_norm": 5, "curv": 6, "s_nx": 7, "s_ny": 8, "s_nz": 9}
+ARRAYS = {"pdf": 0, "phi": 1, "cn_x": 2, "cn_y": 3, "cn_z": 4, "c_norm": 5, "curv": 6, "s_nx": 7, "s_ny": 8, "s_nz": 9,
+          "w_in": 10, "convec": 11}
 
 
 class Oracle:
@@ -121,7 +123,13 @@ class Oracle:
     def array(self, name):
         n = C.c_int64()
         ptr = lib().oracle_array(self.h, ARRAYS[name], C.byref(n))
+        if n.value == 0:
+            return np.zeros(0)
         a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(n.value,))
+        if name == "w_in":
+            return a.reshape(self.ny + 2, self.nx + 2)
+        if name == "convec":
+            return a.reshape(39, self.ny + 2, self.nx + 2)
         halo = {"pdf": 1, "phi": 4, "curv": 1, "s_nx": 4, "s_ny": 4, "s_nz": 4}.get(name, 2)
         if name == "pdf":
             return a.reshape((2, 19) + self.shape(1))

@@ -7,12 +7,16 @@ from mflbm import cases, config
 
 
 def small_drainage(n=(24, 24, 32), seed=7, radius=4.0, porosity=0.45, jper=1, theta=45.0, nu_ratio=1.0,
-                   body_force=5e-4, buffer=3):
-    """Sphere-pack drainage case with empty inlet/outlet buffers (reference axis layout)."""
+                   body_force=5e-4, buffer=3, inlet=2, outlet=2, ca=0.05, plate=0, plate_z=0):
+    """Sphere-pack drainage case with empty inlet/outlet buffers (reference axis layout).
+    inlet / outlet: 1 velocity inlet / convective outlet (mean inlet velocity Ca * gamma / nu1),
+    2 Zou-He pressure; plate: porous plate blocking fluid 1 or 2 at z = plate_z."""
     nx, ny, nz = n
     nu2 = cases.nu_from_tau(0.7)
     ctrl = cases.drainage_control(n=n, theta=theta, nu1=nu_ratio * nu2, nu2=nu2, gamma=0.01,
                                   body_force=body_force, interface_z0=buffer + 1.0, jper=jper)
+    ctrl.update(inlet_BC=inlet, outlet_BC=outlet, capillary_number=ca, porous_plate_cmd=plate,
+                Z_porous_plate=plate_z)
     geo = cases.sphere_geometry(n, radius, porosity, seed, jper=jper)
     geo[:buffer] = 0
     geo[nz - buffer:] = 0

@@ -38,8 +38,9 @@ def test_valid_inputs_pass(tmp_path):
     ({"theta": 100.0}, "contact angle is larger than 90 degrees"),
     ({"iper": 1}, "X direction periodic BC enabled"),
     ({"jper": 0}, "non-slip BC not applied at y"),
-    ({"porous_plate_cmd": 1}, "porous plate"),
-    ({"inlet_BC": 1}, "velocity inlet"),
+    ({"porous_plate_cmd": 3}, "porous_plate_cmd must be 0, 1 or 2"),
+    ({"inlet_BC": 2, "outlet_BC": 1}, "Inlet pressure + outlet convective BC is not supported"),
+    ({"modify_geometry_cmd": 1}, "modify_geometry_cmd"),
 ])
 def test_rejected_controls(tmp_path, change, message):
     ctrl, geo = small_case()

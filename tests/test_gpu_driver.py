@@ -142,3 +142,71 @@ def test_driver_steady_state_monitors(tmp_path, option, series):
     assert open(os.path.join(root, "job_status.txt")).read() == "simulation_done\n"
     rows = np.loadtxt(os.path.join(root, "results/out1.output", series), ndmin=2)
     assert rows[-1, 1] < 1e-3 and rows[-1, 0] < 20000
+
+
+def test_driver_velocity_inlet_convective_outlet_and_restart(tmp_path):
+    """Velocity inlet + convective outlet through the driver: the injected-volume timers, a
+    checkpoint that carries the outlet state, and a bitwise restart from it."""
+    root = str(tmp_path)
+    nx, ny, nz = N
+    ctrl, geo = case(99)
+    ctrl.update(inlet_BC=1, outlet_BC=1, capillary_number=0.05, target_inject_pore_volume=0.1,
+                d_vol_monitor=0.02, d_vol_animation=-1.0, d_vol_detail=-1.0)
+    write_case(root, ctrl, geo)
+    say("driver: velocity inlet + convective outlet, injected-volume timers")
+    out = run_driver(root)
+    pore_sum = int(out.split("Total number of pore nodes = ")[1].split()[0])
+    uin = 0.05 * ctrl["surface_tension"] / ctrl["fluid1_viscosity"]
+    flowrate = uin * ((nx - 2.0) * (ny - 2.0))
+
+    def even(t):
+        return t + 1 if t % 2 == 1 else t
+
+    ntime_max, t_mon = even(int(0.1 * pore_sum / flowrate)), even(int(0.02 * pore_sum / flowrate))
+    assert f"ntime_max = {ntime_max}" in out and f"Monitor timer is modified based on injected volume: {t_mon}" in out
+    last = 1 + ntime_max
+    assert f"Simulation ended after\t{last} iterations" in out
+    sat = np.loadtxt(os.path.join(root, "results/out1.output/saturation.dat"), ndmin=2)
+    assert list(sat[:, 0]) == list(range(t_mon, last + 1, t_mon))
+
+    say(f"reference: python API, {last} steps")
+    p = config.derive(ctrl)
+    ref = gpu.Solver(p, geo, gpu.FP64)
+    ref.setup()
+    ref.run(1, last)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    s1, s4, tail = (nx + 2) * (ny + 2) * (nz + 2), (nx + 8) * (ny + 8) * (nz + 8), 39 * (nx + 2) * (ny + 2)
+    assert nt == last + 1 and body.size == 2 + 38 * s1 + s4 + tail
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    assert np.array_equal(body[2 + 38 * s1 + s4:], ref.download("convec").ravel())
+
+    ctrl2 = dict(ctrl, target_inject_pore_volume=-1.0, max_time_step=99)
+    write_case(root, ctrl2, geo, status="continue_simulation")
+    say("driver: continue_simulation, 100 more steps")
+    out = run_driver(root)
+    assert f"Simulation ended after\t{last + 100} iterations" in out
+    ref.run(last + 1, 100)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == last + 101
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    assert np.array_equal(body[2 + 38 * s1 + s4:], ref.download("convec").ravel())
+    ref.close()
+
+
+def test_driver_porous_plate_run(tmp_path):
+    """porous_plate_cmd / Z_porous_plate from the control file reach the step: the run's
+    checkpoint equals the Python API run of the same plate."""
+    root = str(tmp_path)
+    nx, ny, nz = N
+    ctrl, geo = case(59)
+    ctrl.update(porous_plate_cmd=1, Z_porous_plate=nz - 2)
+    write_case(root, ctrl, geo)
+    say("driver: porous plate blocking fluid 1 near the outlet, 60 steps")
+    run_driver(root)
+    ref = gpu.Solver(config.derive(ctrl), geo, gpu.FP64)
+    ref.setup()
+    ref.run(1, 60)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    assert nt == 61 and np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    ref.close()

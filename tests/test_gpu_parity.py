@@ -46,6 +46,11 @@ def compare_geometry(o, g):
 def compare_state(o, g, tol=0.0, tag=""):
     assert_same(g.download("pdf"), o.array("pdf"), tol, what=f"{tag} pdf")
     assert_same(g.download("phi"), o.array("phi"), tol, what=f"{tag} phi")
+    open_z = o.p["kper"] == 0 and o.p["wall_z_min"] == 0 and o.p["wall_z_max"] == 0
+    if open_z and o.p["inlet_bc"] == 1:
+        assert_same(g.download("w_in"), o.array("w_in"), tol, what=f"{tag} w_in")
+    if open_z and o.p["outlet_bc"] == 1:
+        assert_same(g.download("convec"), o.array("convec"), tol, what=f"{tag} convective outlet state")
     for n in ("cn_x", "cn_y", "cn_z", "c_norm"):
         assert_same(g.download(n), o.array(n), tol, what=f"{tag} {n}")
     m = fluid_interior_mask(o)
@@ -56,6 +61,13 @@ CASES = {
     "drainage_yper": lambda: small_drainage(n=(24, 24, 32), seed=7, jper=1),
     "drainage_ywalls": lambda: small_drainage(n=(20, 24, 28), seed=11, jper=0, theta=30.0, nu_ratio=0.1),
     "drop_periodic_yz": lambda: periodic_drop(n=(24, 20, 22), radius=6.0, geo_seed=3, body_force=1e-5),
+    # velocity inlet (src/main_iteration_GPU.cu:1008-1077) and convective outlet (:1245-1357)
+    "velocity_in_convective_out": lambda: small_drainage(n=(20, 18, 28), seed=5, jper=0, inlet=1, outlet=1),
+    "velocity_in_zouhe_out_yper": lambda: small_drainage(n=(22, 20, 26), seed=8, jper=1, inlet=1, outlet=2),
+    "no_inlet_bc_convective_out": lambda: small_drainage(n=(20, 20, 24), seed=9, jper=1, inlet=0, outlet=1),
+    # porous plate (src/main_iteration_GPU.cu:1744-1882) across the sphere pack / in the inlet buffer
+    "plate_blocks_fluid1": lambda: small_drainage(n=(20, 22, 28), seed=6, jper=1, plate=1, plate_z=14),
+    "plate_blocks_fluid2": lambda: small_drainage(n=(18, 20, 26), seed=4, jper=0, plate=2, plate_z=2),
 }
 
 
@@ -277,3 +289,43 @@ def test_fp32_pdf_round_trip():
     g.upload("pdf", a)
     b = g.download("pdf")
     assert np.abs(b.astype(np.float64) - a).max() <= 2e-8
+
+
+@pytest.mark.parametrize("name, kw", [
+    ("velocity_in_convective_out_300", dict(n=(20, 18, 28), seed=5, jper=0, inlet=1, outlet=1, ca=0.1)),
+    ("plate_at_inlet_plane_300", dict(n=(18, 20, 24), seed=3, jper=1, plate=2, plate_z=1)),
+    ("plate_at_outlet_plane_300", dict(n=(18, 18, 24), seed=2, jper=0, inlet=1, outlet=1, plate=1, plate_z=24)),
+])
+def test_open_boundaries_300_steps_bitwise(name, kw):
+    """Longer runs of the open boundaries through the step graphs: the injected front
+    moves into the pack and the outlet state turns over."""
+    geo, p = small_drainage(**kw)
+    say(f"case={name} 300 steps")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    o.run(1, 300)
+    g.run(1, 300)
+    compare_state(o, g, tag=name)
+
+
+def test_fp32_velocity_inlet_tracks_fp64():
+    """FP32 storage with the velocity inlet and convective outlet: the injected saturation
+    and the velocity field follow the FP64 run."""
+    geo, p = small_drainage(n=(20, 18, 28), seed=5, jper=0, inlet=1, outlet=1, ca=0.1)
+    say("fp32 vs fp64, velocity inlet + convective outlet, 400 steps")
+    a = gpu.Solver(p, geo, gpu.FP64)
+    b = gpu.Solver(p, geo, gpu.FP32)
+    for s in (a, b):
+        s.setup()
+        s.run(1, 400)
+    _, ua, va, wa = a.macro()
+    _, ub, vb, wb = b.macro()
+    err = np.sqrt(((ua - ub) ** 2 + (va - vb) ** 2 + (wa - wb) ** 2).sum() / (ua ** 2 + va ** 2 + wa ** 2).sum())
+    ma, mb = a.monitor(), b.monitor()
+    sa = ma["vol1"].sum() / (ma["vol1"].sum() + ma["vol2"].sum())
+    sb = mb["vol1"].sum() / (mb["vol1"].sum() + mb["vol2"].sum())
+    say(f"velocity rel L2 {err:.2e}, saturation {sa:.6f} vs {sb:.6f}")
+    assert err < 1e-3 and abs(sa - sb) < 1e-4
+    a.close()
+    b.close()
