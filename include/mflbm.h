@@ -141,6 +141,13 @@ int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w);
 /* Monitor reductions of src/Monitor.cpp:17-80 on the device in FP64. */
 int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out);
 
+/* Steady criterion of a single-phase solve (no counterpart in the reference, whose steady
+ * monitors follow saturation, phase field or capillary pressure): *change = |u - u_prev| /
+ * |u| over the fluid nodes, FP64 L2 norms of the compute_macro_vars velocity, u_prev being
+ * the velocity at the previous call; -1 on the first call after a (re)initialisation or
+ * a PDF upload.  Reduced on the device in a fixed order. */
+int mflbm_velocity_change(mflbm_handle* h, double* change);
+
 /* Waits for all work of the handle. */
 int mflbm_sync(mflbm_handle* h);
 

@@ -65,6 +65,8 @@ struct mflbm_handle {
     // monitor scratch
     double *mon_part = nullptr, *mon_out = nullptr;
     int mon_nsplit = 0;
+    double *vprev = nullptr, *vpart = nullptr;  // mflbm_velocity_change: 3 nf previous velocity, block sums
+    bool vprev_valid = false;
     // captured step graphs: [0] even step, [1] odd step
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     double rho_ref = 1.0;  // FP32 storage reference density (ShiftedStorage)
@@ -227,7 +229,7 @@ void free_all(mflbm_handle* h) {
     for (int g = 0; g < 2; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
-                    h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out,
+                    h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
                     h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
                     h->ext_node, h->ext_off, h->zstart,
                     h->tag};
@@ -236,6 +238,7 @@ void free_all(mflbm_handle* h) {
     h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
+    h->vprev = h->vpart = nullptr;
     h->cells = h->aux = nullptr;
     h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->pmap = h->ext_node = h->ext_off = h->zstart = nullptr;
     h->w_in = h->conv = nullptr;
@@ -590,6 +593,7 @@ int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0) {
 static int init_populations(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w,
                             bool relaxed) {
     if (!h) return -1;
+    h->vprev_valid = false;
     const int64_t n = h->d.S1;
     const double* src[4] = {rho, u, v, w};
     double* dv[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -741,7 +745,10 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
     } else {
         HIPCHK(h, hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice));
     }
-    if (field == MFLBM_PDF) cells_transfer(h, false);
+    if (field == MFLBM_PDF) {
+        cells_transfer(h, false);
+        h->vprev_valid = false;
+    }
     if (field == MFLBM_CN_X || field == MFLBM_CN_Y || field == MFLBM_CN_Z || field == MFLBM_C_NORM || field == MFLBM_CURV)
         aux_refresh(h);
     HIPCHK(h, hipGetLastError());
@@ -796,6 +803,28 @@ int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out) {
     }
     out->umax2 = umax; out->usq1 = u1; out->usq2 = u2;
     out->n_fluid = h->n_fluid;
+    return 0;
+}
+
+int mflbm_velocity_change(mflbm_handle* h, double* change) {
+    if (!h || !change) return -1;
+    const int64_t nb = (h->n_fluid + 255) / 256;
+    if (!h->vprev) {
+        HIPCHK(h, hipMalloc(&h->vprev, (size_t)(3 * h->n_fluid + 1) * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->vpart, (size_t)(2 * nb + 2) * sizeof(double)));
+        h->vprev_valid = false;
+    }
+    double* out = h->vpart + 2 * nb;  // the two totals follow the block sums
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::velocity_change(sparse<double>(h), consts<double>(h), h->vprev, h->vpart, out, h->stream);
+    else
+        Launch<float>::velocity_change(sparse<float>(h), consts<float>(h), h->vprev, h->vpart, out, h->stream);
+    HIPCHK(h, hipGetLastError());
+    double tot[2];
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(tot, out, sizeof(tot), hipMemcpyDeviceToHost));
+    *change = h->vprev_valid ? std::sqrt(tot[0]) / std::max(std::sqrt(tot[1]), 1e-300) : -1.0;
+    h->vprev_valid = true;
     return 0;
 }
 

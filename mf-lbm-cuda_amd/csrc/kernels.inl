@@ -844,5 +844,59 @@ __global__ void k_monitor_combine(const double* __restrict__ part, int nsplit, i
     }
 }
 
+// Relative change of the velocity field since the last call (steady criterion of the warm
+// start): per block, FP64 sums of |u - u_prev|^2 and |u|^2 over its fluid nodes, and u_prev
+// replaced by u; k_sum_pairs then adds the block sums in block order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void k_velocity_change(Sparse<T> S, StepConst<T> P, double* __restrict__ prev,
+                                                         double* __restrict__ part) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double d2 = 0.0, u2 = 0.0;
+    if (t < S.nf) {
+        double r, u, v, w;
+        macro_node(S, P, t, r, u, v, w);
+        const double du = u - prev[t], dv = v - prev[S.nf + t], dw = w - prev[2 * (int64_t)S.nf + t];
+        d2 = du * du + dv * dv + dw * dw;
+        u2 = u * u + v * v + w * w;
+        prev[t] = u;
+        prev[S.nf + t] = v;
+        prev[2 * (int64_t)S.nf + t] = w;
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = d2;
+    red[1][threadIdx.x] = u2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) part[2 * (int64_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[0..1] = sums of the n pairs in part; one block, each thread a fixed stride.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ part, int n, double* __restrict__ out) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += part[2 * (int64_t)i];
+        b += part[2 * (int64_t)i + 1];
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
 }  // namespace dev
 }  // namespace mflbm

@@ -29,6 +29,9 @@ struct Launch {
                         bool relaxed, double rho_ref, hipStream_t s);
     static void macro(const Sparse<T>& S, const StepConst<T>& P, const Dims& d, double* rho, double* u, double* v,
                       double* w, hipStream_t s);
+    // relative velocity change since the last call; part: 2 * ceil(nf / 256) doubles, out: 2
+    static void velocity_change(const Sparse<T>& S, const StepConst<T>& P, double* prev, double* part, double* out,
+                                hipStream_t s);
     static void monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi, const Dims& d, const int32_t* zstart,
                         int nsplit, double* part, double* out, hipStream_t s);
     static void to_real(const double* src, T* dst, int64_t n, hipStream_t s);

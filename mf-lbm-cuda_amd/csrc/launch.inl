@@ -95,6 +95,14 @@ void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_
 }
 
 template <typename T>
+void Launch<T>::velocity_change(const Sparse<T>& S, const StepConst<T>& P, double* prev, double* part, double* out,
+                                hipStream_t s) {
+    const int nb = (int)blocks(S.nf);
+    if (nb > 0) dev::k_velocity_change<T><<<nb, 256, 0, s>>>(S, P, prev, part);
+    dev::k_sum_pairs<T><<<1, 256, 0, s>>>(part, nb, out);
+}
+
+template <typename T>
 void Launch<T>::aux_refresh(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s) {
     if (L.n_fluid > 0) dev::k_aux_refresh<T><<<blocks(L.n_fluid), 256, 0, s>>>(F, P, L.fluid, L.n_fluid, aux);
 }

@@ -26,7 +26,7 @@ HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv"
 EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_equilibrium",
            "mflbm_init_relaxed",
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
-           "mflbm_macro", "mflbm_monitor", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe",
+           "mflbm_macro", "mflbm_monitor", "mflbm_velocity_change", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe",
            "mflbm_ensemble_create", "mflbm_ensemble_destroy", "mflbm_ensemble_last_error",
            "mflbm_ensemble_fluid_nodes", "mflbm_ensemble_solve", "mflbm_ensemble_fields"]
 
@@ -89,6 +89,7 @@ def load():
     L.mflbm_upload.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     L.mflbm_macro.argtypes = [vp, dp, dp, dp, dp]
     L.mflbm_monitor.argtypes = [vp, C.POINTER(MonitorOut)]
+    L.mflbm_velocity_change.argtypes = [vp, C.POINTER(C.c_double)]
     L.mflbm_sync.argtypes = [vp]
     L.mflbm_count.argtypes = [vp, C.c_int32]
     L.mflbm_count.restype = C.c_int64
@@ -233,6 +234,12 @@ class Solver:
         rho, u, v, w = (np.zeros(shp) for _ in range(4))
         self._check(load().mflbm_macro(self.h, _dp(rho), _dp(u), _dp(v), _dp(w)))
         return rho, u, v, w
+
+    def velocity_change(self) -> float:
+        """|u - u_prev| / |u| since the previous call (-1 after an initialisation)"""
+        c = C.c_double()
+        self._check(load().mflbm_velocity_change(self.h, C.byref(c)))
+        return c.value
 
     def monitor(self) -> dict:
         nz = self.nz

@@ -224,6 +224,8 @@ class UNet:
         for c in (cin, cout, c0, c1, c2, c3):
             if c % 8:
                 raise ValueError("channel counts must be multiples of 8")
+        if c3 + c2 > 192:
+            raise ValueError("the data gradient of the first decoder layer has c3 + c2 outputs, at most 192")
         rng = np.random.default_rng(seed)
         shapes = {"e1": (cin, c0), "e2": (c0, c0), "e3": (c0, c1), "e4": (c1, c1), "e5": (c1, c2), "e6": (c2, c2),
                   "e7": (c2, c3), "e8": (c3, c3), "d9": (c3 + c2, c2), "d10": (c2 + c1, c1), "d11": (c1 + c0, c0),

@@ -36,7 +36,17 @@ squared, independent of the driving force) and the density perturbation scaled b
 Steady criterion: the relative L2 change of the velocity field over `check` steps falls
 below `tol` (default 1e-5 per 100 steps, about 1e-7 per step).
 
+Fine grids (`--fine`): the network runs on the geometry coarsened by two (solid where at
+least half of a 2x2x2 block is solid) and its fields are interpolated back, the velocity
+times 4 and the density perturbation times 2 (the targets are a length squared and a length
+in lattice units, and the lattice unit halves).  The held-out fine geometries are the
+two-fold refinements of held-out coarse packs.  A warm start whose velocity still changes by
+more than `fallback_residual` per step (relative L2, over the 100 steps ending at step
+`fallback_step`) is abandoned for the relaxed start from rest (the method without the
+network), and the steps already taken count against it.
+
     python -m mflbm.warmstart --train 48 --eval 8 --epochs 30
+    python -m mflbm.warmstart --n 128 --fine --train 96 --eval 2 --epochs 20
 """
 from __future__ import annotations
 
@@ -112,26 +122,51 @@ def relaxed_mean_density(geo):
     return (nf + w_std) / (nf + r_rel)
 
 
-def run_to_steady(sim, params, init=None, relaxed=False, tol=1e-5, check=100, max_steps=60000):
+PROBE_WINDOW = 100
+
+
+def run_to_steady(sim, params, init=None, relaxed=False, tol=1e-5, check=100, max_steps=60000, probe=None,
+                  log=None, info=None):
     """Initialise from init = (rho, u, v, w) on the s1 grid (None = the reference's rest
     state) with the D3Q19 or, relaxed, the collision's rest weights, and step until the
     relative L2 change of the velocity over `check` steps is below tol.
+    probe = (step, residual): give up, returning (steps, None, None), if the residual at
+    `step` -- the relative velocity change per step over the PROBE_WINDOW steps ending
+    there -- is above `residual` (stored in info["residual"]).  The steady test resumes
+    from the probe step.
     Returns (steps, velocity (3, nz, ny, nx), density (nz, ny, nx))."""
     sim.init_phi(params["initial_fluid_distribution_option"], params["interface_z0"])
     sim.init_equilibrium(*(init or (None, None, None, None)), relaxed=relaxed)
     sim.color_gradient()
-    prev = None
+    sim.velocity_change()  # the change is measured on the device from here on
     t = 1
     while t <= max_steps:
+        if probe and t - 1 < probe[0] <= t - 1 + check and probe[0] - PROBE_WINDOW >= t - 1:
+            pre = probe[0] - PROBE_WINDOW - (t - 1)
+            if pre:
+                sim.run(t, pre)
+            sim.velocity_change()
+            sim.run(t + pre, PROBE_WINDOW)
+            residual = sim.velocity_change() / PROBE_WINDOW
+            t = probe[0] + 1
+            if info is not None:
+                info["residual"] = residual
+            if log:
+                log(f"  step {probe[0]}: residual {residual:.3e} per step")
+            if residual > probe[1]:
+                return t - 1, None, None
+            continue
         sim.run(t, check)
         t += check
-        rho, u, v, w = sim.macro()
-        vel = np.stack([u[1:-1, 1:-1, 1:-1], v[1:-1, 1:-1, 1:-1], w[1:-1, 1:-1, 1:-1]])
-        if prev is not None:
-            change = np.linalg.norm(vel - prev) / max(np.linalg.norm(vel), 1e-300)
-            if change < tol:
-                break
-        prev = vel
+        change = sim.velocity_change()
+        if log and (t - 1) % (50 * check) == 0:
+            log(f"  step {t - 1}: velocity change {change:.3e}")
+        if probe and t - 1 == probe[0] and change > probe[1]:
+            return t - 1, None, None
+        if change < tol:
+            break
+    rho, u, v, w = sim.macro()
+    vel = np.stack([u[1:-1, 1:-1, 1:-1], v[1:-1, 1:-1, 1:-1], w[1:-1, 1:-1, 1:-1]])
     return t - 1, vel, rho[1:-1, 1:-1, 1:-1]
 
 
@@ -139,6 +174,26 @@ def permeability(vel, params):
     """k = nu <w> / F (Darcy velocity over the whole cross-section)"""
     nu = 1.0 / params["la_nui1"]
     return float(vel[2].mean() * nu / BODY_FORCE)
+
+
+def coarsen(geo):
+    """solid where at least half of each 2x2x2 block is solid (exact for refine(g))"""
+    nz, ny, nx = geo.shape
+    b = geo.astype(np.int16).reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).sum(axis=(1, 3, 5))
+    return (b >= 4).astype(np.int8)
+
+
+def refine(geo):
+    """each node becomes a 2x2x2 block"""
+    return np.asarray(geo, np.int8).repeat(2, 0).repeat(2, 1).repeat(2, 2)
+
+
+def fine_case(seed, n_coarse):
+    """the two-fold refinement of single_phase_case(seed, n_coarse), same body force"""
+    gc, _, meta = single_phase_case(seed, n_coarse)
+    n = 2 * n_coarse
+    ctrl = cases.periodic_control(n=(n, n, n), body_force=BODY_FORCE, option=1, interface_z0=float(n + 10))
+    return refine(gc), config.derive(ctrl), dict(meta, radius=2 * meta["radius"])
 
 
 def to_s1(field):
@@ -166,7 +221,11 @@ def generate(seeds, n=64, log=print, chunk=512):
         geos, metas = ensemble.sphere_pack_ensemble(part, n)
         _, params, _ = single_phase_case(part[0], n)
         ens = ensemble.Ensemble(params, geos)
-        out = ens.solve(check=100, tol=1e-5, max_steps=60000)
+        for taken in range(0, 60000, 2000):  # in slices, to report progress
+            out = ens.solve(check=100, tol=1e-5, max_steps=2000)
+            if (out["steps"] > 0).all():
+                break
+            log(f"  {taken + 2000} steps: {(out['steps'] > 0).sum()} of {len(part)} domains steady")
         fields = ens.fields()
         rho_ref = ens.rho_ref.copy()
         ens.close()
@@ -191,9 +250,21 @@ class WarmStart:
     what both give)."""
 
     def __init__(self, widths=(16, 32, 64, 128), seed=0, rho_weight=16.0):
+        self.widths = tuple(widths)
         self.net = nn.UNet(widths=widths, cin=8, cout=8, n_out=4, seed=seed)
         self.scale = np.ones(4)
         self.rho_weight = rho_weight
+
+    def save(self, path):
+        np.savez(path, scale=self.scale, widths=np.array(self.widths), rho_weight=self.rho_weight, **self.net.state())
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path) as z:
+            ws = cls(widths=tuple(int(c) for c in z["widths"]), rho_weight=float(z["rho_weight"]))
+            ws.scale = z["scale"].copy()
+            ws.net.load_state({k: z[k] for k in z.files if k.endswith(".w") or k.endswith(".b")})
+        return ws
 
     def _tensors(self, geo, target=None):
         x = nn.DevArray.from_numpy(features(geo), "bf16")
@@ -211,7 +282,11 @@ class WarmStart:
         self.scale[:3] = self.scale[2]  # one velocity scale, set by the flow-direction mean
         self.scale[3] /= np.sqrt(self.rho_weight)
         self.scale = np.maximum(self.scale, 1e-12)
-        tensors = [self._tensors(g, t) for g, t, _, _ in data]
+        tensors = []
+        for i, (g, t, _, _) in enumerate(data):
+            tensors.append(self._tensors(g, t))
+            if (i + 1) % 32 == 0:
+                log(f"  network inputs: {i + 1} of {len(data)}")
         hist = []
         rng = np.random.default_rng(0)
         for ep in range(epochs):
@@ -239,6 +314,85 @@ class WarmStart:
         rbar = relaxed_mean_density(geo)
         rho = np.where(fluid, rbar + drho - drho[fluid].mean(), rbar)
         return tuple(to_s1(a) for a in (rho, *vel))
+
+
+    def predict_fine(self, geo, params):
+        """(rho, u, v, w) on the s1 grid of a geometry twice as fine as the network's grid"""
+        from scipy import ndimage
+        y = self.net.forward(self._tensors(coarsen(geo))[0])["y"].numpy()
+        up = lambda a: ndimage.zoom(a, 2, order=1, grid_mode=True, mode="nearest")
+        nu = 1.0 / params["la_nui1"]
+        fluid = ~solid_with_walls(geo)
+        vel = [np.where(fluid, up(y[..., c]) * (4.0 * self.scale[c] * BODY_FORCE / nu), 0.0) for c in range(3)]
+        vel[2] = vel[2] - 0.5 * BODY_FORCE * fluid
+        drho = up(y[..., 3]) * (2.0 * self.scale[3] * 3.0 * BODY_FORCE)
+        rbar = relaxed_mean_density(geo)
+        rho = np.where(fluid, rbar + drho - drho[fluid].mean(), rbar)
+        return tuple(to_s1(a) for a in (rho, *vel))
+
+
+def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=500, tol=5e-5,
+                  cold_max_steps=400000, fallback_step=500, fallback_residual=1e-4, starts=("cold", "relaxed", "warm")):
+    """Held-out geometries twice as fine as the network's grid: the reference's FP64 cold
+    start (the judge), the relaxed start from rest and the learned warm start in `precision`,
+    with the fallback rule on the warm start."""
+    rows = []
+    for s in seeds:
+        geo, params, meta = fine_case(s, n_coarse)
+        row = dict(meta, n=geo.shape[0])
+        fluid = ~solid_with_walls(geo)
+        ref = None
+        if "cold" in starts:
+            sim = gpu.Solver(params, geo, gpu.FP64)
+            t0 = time.perf_counter()
+            row["cold_steps"], cold, rho_cold = run_to_steady(sim, params, tol=tol, check=check,
+                                                              max_steps=cold_max_steps, log=log)
+            row["cold_s"] = time.perf_counter() - t0
+            sim.close()
+            row["k_cold"] = permeability(cold, params)
+            ref = (cold, rho_cold)
+            log(f"held-out {s} ({geo.shape[0]}^3): cold {row['cold_steps']} steps {row['cold_s']:.1f} s, "
+                f"k {row['k_cold']:.5f}")
+        sim = gpu.Solver(params, geo, precision)
+        for name in [n for n in ("relaxed", "warm") if n in starts]:
+            t0 = time.perf_counter()
+            if name == "warm":
+                init = ws.predict_fine(geo, params)
+                row["predict_s"] = time.perf_counter() - t0
+                if ref is not None:
+                    pr = [a[1:-1, 1:-1, 1:-1] for a in init]
+                    vel_p = np.stack(pr[1:])
+                    vel_p[2] += 0.5 * BODY_FORCE * fluid
+                    row["velocity_rel_err"] = float(np.linalg.norm((vel_p - ref[0])[:, fluid]) /
+                                                    np.linalg.norm(ref[0][:, fluid]))
+                    d_ref = ref[1][fluid] - ref[1][fluid].mean()
+                    d_pre = pr[0][fluid] - pr[0][fluid].mean()
+                    row["density_rel_err"] = float(np.linalg.norm(d_pre - d_ref) / max(np.linalg.norm(d_ref), 1e-300))
+                    del pr, vel_p
+                info = {}
+                steps, vel, _ = run_to_steady(sim, params, init, relaxed=True, tol=tol, check=check,
+                                              max_steps=cold_max_steps, probe=(fallback_step, fallback_residual), log=log,
+                                              info=info)
+                row["residual_at_probe"] = info.get("residual")
+                row["fallback"] = vel is None
+                if vel is None:
+                    lost = steps
+                    steps, vel, _ = run_to_steady(sim, params, rest_state(geo), relaxed=True, tol=tol, check=check,
+                                                  max_steps=cold_max_steps, log=log)
+                    steps += lost
+            else:
+                steps, vel, _ = run_to_steady(sim, params, rest_state(geo), relaxed=True, tol=tol, check=check,
+                                              max_steps=cold_max_steps, log=log)
+            row[f"{name}_steps"] = steps
+            row[f"{name}_s"] = time.perf_counter() - t0
+            row[f"k_{name}"] = permeability(vel, params)
+            if ref is not None:
+                row[f"k_{name}_rel_diff"] = abs(row[f"k_{name}"] - row["k_cold"]) / abs(row["k_cold"])
+            log(f"held-out {s}: {name} {steps} steps {row[f'{name}_s']:.1f} s, k {row[f'k_{name}']:.5f}"
+                + (f", fallback {row['fallback']}" if name == "warm" else ""))
+        sim.close()
+        rows.append(row)
+    return rows
 
 
 def evaluate(ws, seeds, n=64, precision=gpu.FP32, log=print):
@@ -305,16 +459,39 @@ def main(argv=None):
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32", help="of the relaxed and warm solves")
     ap.add_argument("--rho-weight", type=float, default=16.0, help="loss weight of the density channel")
     ap.add_argument("--out", default=None, help="write the JSON summary here as well")
+    ap.add_argument("--widths", default="16,32,64,128", help="channels of the four scales")
+    ap.add_argument("--fine", action="store_true", help="evaluate on geometries twice as fine as --n")
+    ap.add_argument("--save", default=None, help="write the trained network (.npz) here")
+    ap.add_argument("--load", default=None, help="skip data and training: evaluate this network (.npz)")
+    ap.add_argument("--starts", default="cold,relaxed,warm", help="fine evaluation: which starts to run")
+    ap.add_argument("--eval-seeds", default=None, help="comma-separated held-out seeds (default 1000, 1001, ...)")
+    ap.add_argument("--cold-max-steps", type=int, default=400000)
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    eval_seeds = ([int(v) for v in args.eval_seeds.split(",")] if args.eval_seeds
+                  else list(range(1000, 1000 + args.eval)))
+    prec = gpu.FP32 if args.precision == "fp32" else gpu.FP64
+    if args.load:
+        ws = WarmStart.load(args.load)
+        log(f"network parameters: {ws.net.n_params()} (loaded)")
+        rows = evaluate_fine(ws, eval_seeds, args.n, prec, log, starts=tuple(args.starts.split(",")),
+                             cold_max_steps=args.cold_max_steps)
+        text = json.dumps({"n": args.n, "fine": True, "params": ws.net.n_params(), "eval": rows})
+        if args.out:
+            with open(args.out, "w") as fh:
+                fh.write(text + "\n")
+        print(text, flush=True)
+        return
     t0 = time.perf_counter()
     data = generate(range(args.train), args.n, log)
     t_data = time.perf_counter() - t0
-    ws = WarmStart(rho_weight=args.rho_weight)
+    ws = WarmStart(widths=tuple(int(c) for c in args.widths.split(",")), rho_weight=args.rho_weight)
     log(f"network parameters: {ws.net.n_params()}")
     t0 = time.perf_counter()
     hist = ws.train(data, args.epochs, args.lr, log)
     t_train = time.perf_counter() - t0
+    if args.save:
+        ws.save(args.save)
     # fit on a few training samples, in the same terms as the held-out field errors
     fit = []
     for geo, target, params, _ in data[:4]:
@@ -329,11 +506,17 @@ def main(argv=None):
         fit.append((float(np.linalg.norm((vel_p - vel)[:, fluid]) / np.linalg.norm(vel[:, fluid])),
                     float(np.linalg.norm(d_pre - d_ref) / np.linalg.norm(d_ref))))
     log("training-sample field errors (u, rho): " + ", ".join(f"({a:.3f}, {b:.3f})" for a, b in fit))
-    prec = gpu.FP32 if args.precision == "fp32" else gpu.FP64
-    rows = evaluate(ws, range(1000, 1000 + args.eval), args.n, prec, log)
+    del data
     res = {"n": args.n, "train_samples": args.train, "epochs": args.epochs, "params": ws.net.n_params(),
            "precision": args.precision, "data_seconds": t_data, "train_seconds": t_train, "loss_first": hist[0],
-           "loss_last": hist[-1], **summarise(rows), "eval": rows}
+           "loss_last": hist[-1], "train_fit": fit}
+    if args.fine:
+        rows = evaluate_fine(ws, eval_seeds, args.n, prec, log, starts=tuple(args.starts.split(",")),
+                             cold_max_steps=args.cold_max_steps)
+        res.update(fine=True, eval=rows)
+    else:
+        rows = evaluate(ws, eval_seeds, args.n, prec, log)
+        res.update(**summarise(rows), eval=rows)
     text = json.dumps(res)
     if args.out:
         with open(args.out, "w") as fh:

@@ -329,3 +329,28 @@ def test_fp32_velocity_inlet_tracks_fp64():
     assert err < 1e-3 and abs(sa - sb) < 1e-4
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [gpu.FP64, gpu.FP32])
+def test_velocity_change_matches_host_norms(precision):
+    """mflbm_velocity_change against the same norms of mflbm_macro's velocity on the host."""
+    from mflbm import warmstart
+    geo, p, _ = warmstart.single_phase_case(3, 32)
+    s = gpu.Solver(p, geo, precision)
+    s.setup()
+    assert s.velocity_change() == -1.0
+    vel = lambda: np.stack(s.macro()[1:])
+    u0 = vel()
+    s.run(1, 60)
+    u1 = vel()
+    c = s.velocity_change()
+    ref = np.linalg.norm(u1 - u0) / np.linalg.norm(u1)
+    say(f"velocity change {c:.15e} host {ref:.15e}")
+    assert c > 0 and abs(c - ref) <= 1e-10 * ref
+    s.run(61, 40)
+    u2 = vel()
+    c = s.velocity_change()
+    assert abs(c - np.linalg.norm(u2 - u1) / np.linalg.norm(u2)) <= 1e-10 * c
+    s.upload("pdf", s.download("pdf"))
+    assert s.velocity_change() == -1.0
+    s.close()

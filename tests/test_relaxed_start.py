@@ -88,3 +88,25 @@ def test_relaxed_start_reaches_the_cold_steady_state_sooner():
     k_cold, k_rel = w_cold[1:-1, 1:-1, 1:-1].mean(), w_rel[1:-1, 1:-1, 1:-1].mean()
     assert abs(k_rel / k_cold - 1) < 1e-4
     assert 3 * s_rel <= s_cold, (s_rel, s_cold)
+
+
+def test_coarsen_inverts_refine_and_keeps_the_majority():
+    from mflbm import warmstart
+    rng = np.random.default_rng(3)
+    g = (rng.random((8, 6, 10)) < 0.4).astype(np.int8)
+    f = warmstart.refine(g)
+    assert f.shape == (16, 12, 20) and np.array_equal(warmstart.coarsen(f), g)
+    # a block with 3 of 8 solid nodes coarsens to pore, 4 of 8 to solid
+    b = np.zeros((2, 2, 2), np.int8)
+    b.flat[:3] = 1
+    assert warmstart.coarsen(b)[0, 0, 0] == 0
+    b.flat[3] = 1
+    assert warmstart.coarsen(b)[0, 0, 0] == 1
+
+
+def test_fine_case_is_the_refined_coarse_case():
+    from mflbm import warmstart
+    gc, pc, mc = warmstart.single_phase_case(1000, 16)
+    gf, pf, mf = warmstart.fine_case(1000, 16)
+    assert np.array_equal(warmstart.coarsen(gf), gc) and gf.shape == (32, 32, 32)
+    assert mf["radius"] == 2 * mc["radius"] and pf["force_z"] == pc["force_z"] and pf["nz"] == 32
