@@ -23,8 +23,8 @@
 //   * the convective-outlet tail of the checkpoint is written when that outlet is active
 //     (outlet_BC = 1 on an open z axis); the reference also writes it, unused, when
 //     outlet_BC = 1 on a closed or periodic z axis.  Files with or without it load.
-// Geometry modification, precomputed boundary files and the inlet phase switch are not
-// built; they are rejected up front.
+// Precomputed boundary files (geometry_preprocess_cmd = 1) stop with the reference's own
+// error: its loader is not implemented either.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -177,10 +177,10 @@ void check_control(const Control& c) {
         err.push_back("non-slip BC applied at z = zmin or z = zmax while z direction periodic BC enabled");
     if (c.outlet_bc == 1 && c.inlet_bc == 2) err.push_back("Inlet pressure + outlet convective BC is not supported");
     if (c.porous_plate < 0 || c.porous_plate > 2) err.push_back("porous_plate_cmd must be 0, 1 or 2");
-    if (c.modify_geometry != 0) err.push_back("modify_geometry_cmd = 1 (hard-coded geometry) is not built");
-    if (c.geometry_preprocess != 0)
-        err.push_back("geometry_preprocess_cmd = 1 is not built: boundary info is computed on the device");
-    if (c.change_inlet_phase != 0) err.push_back("change_inlet_fluid_phase_cmd != 0 is not built");
+    // the reference's loader of precomputed boundary info stops with this error itself
+    // (src/Geometry_preprocessing.cpp:413-429); the boundary info is computed on the device
+    if (c.geometry_preprocess == 1) err.push_back("geometry_preprocessing_load is not implemented yet");
+    else if (c.geometry_preprocess != 0) err.push_back("Wrong value of geometry_preprocess_cmd");
     if (c.extreme_large != 0)
         err.push_back("Extreme large simulation domain (extreme_large_sim_cmd = 1) is not supported yet");
     if (c.option < 1 || c.option > 5) err.push_back("initial_fluid_distribution_option must be 1-5");
@@ -267,7 +267,33 @@ std::string dat_path(std::string v) {
 // .dat file: three dimensions (int32 or int64) then nx*ny*nz bytes (1 solid), x fastest.
 // Returns the voxel array of the whole domain; a sample smaller than the domain is padded
 // with solid when both x_max and y_max walls are set (src/Misc.cpp:175-184).
+// modify_geometry_cmd = 1 (src/Misc.cpp:105-136): the reference's sample edit, a sphere of
+// radius ny/4 at the domain centre and everything outside a tube of radius ny/2 between the
+// 10-layer inlet and outlet reservoirs made solid, before the domain walls are added.
+void modify_geometry(const Control& c, std::vector<int8_t>& geo) {
+    const double xc = 0.5 * double(c.nx + 1), yc = 0.5 * double(c.ny + 1), zc = 0.5 * double(c.nz + 1);
+    const double r1 = 0.25 * (double)c.ny, r2 = (double)c.ny * 0.5;
+    const long long buffer = 10;
+    for (long long k = 1; k <= c.nz; ++k)
+        for (long long j = 1; j <= c.ny; ++j)
+            for (long long i = 1; i <= c.nx; ++i) {
+                const double d2 = std::pow(i - xc, 2) + std::pow(j - yc, 2) + std::pow(k - zc, 2);
+                int8_t& w = geo[(size_t)(i - 1) + c.nx * ((size_t)(j - 1) + c.ny * (size_t)(k - 1))];
+                if (d2 < std::pow(r1, 2)) w = 1;
+                if (d2 > std::pow(r2, 2) && k > buffer && k < c.nz - buffer + 1) w = 1;
+            }
+    std::cout << "Internal geometry modified!" << std::endl;
+}
+
+std::vector<int8_t> read_geometry_file(const Control& c);
+
 std::vector<int8_t> read_geometry(const Control& c) {
+    std::vector<int8_t> geo = read_geometry_file(c);
+    if (c.modify_geometry == 1) modify_geometry(c, geo);
+    return geo;
+}
+
+std::vector<int8_t> read_geometry_file(const Control& c) {
     const size_t n = (size_t)c.nx * c.ny * c.nz;
     std::vector<int8_t> geo(n, 0);
     if (c.external_geometry != 1) return geo;  // an empty duct
@@ -354,6 +380,7 @@ private:
     void upload_real(int field, const std::vector<double>& v);
     void geometry_info();
     void injected_volume_timers();
+    void change_inlet_fluid_phase();
     void write_info() const;
     void load_checkpoint(double* force_z, double* rho_in, std::vector<double>* pdf, std::vector<double>* phi,
                          std::vector<double>* conv);
@@ -469,6 +496,28 @@ void Simulation::injected_volume_timers() {
     }
     if (c_.ntime_animation <= 0 || c_.ntime_visual <= 0 || c_.ntime_monitor <= 0)
         throw Fatal("Error: an injected-volume timer is below one step (raise d_vol_* or lower the flow rate)!");
+}
+
+// change_inlet_fluid_phase_cmd 1 / 2 (src/Misc.cpp:277-383, called from src/main.cpp:82-85
+// after the new or loaded state): below interface_z0 every population of the other colour
+// is moved into fluid 1 / fluid 2 (over i = 0 .. nx - 1, j = 0 .. ny + 1, k = 0 .. nz + 1,
+// the reference's ranges); phi is left as it is until the next collision.
+void Simulation::change_inlet_fluid_phase() {
+    std::vector<double> pdf = download_real(MFLBM_PDF, 38 * s1_);
+    const int to = c_.change_inlet_phase == 1 ? 0 : 1, from = 1 - to;
+    for (int64_t k = 0; k <= nz_ + 1; ++k) {
+        if (!((double)k <= c_.interface_z0)) continue;
+        for (int64_t j = 0; j <= ny_ + 1; ++j)
+            for (int64_t i = 0; i <= nx_ - 1; ++i)
+                for (int e = 0; e < 19; ++e) {
+                    double& a = pdf[(size_t)(e + 19 * to) * s1_ + i_s1(i, j, k)];
+                    double& b = pdf[(size_t)(e + 19 * from) * s1_ + i_s1(i, j, k)];
+                    a += b;
+                    b = 0.0;
+                }
+    }
+    upload_real(MFLBM_PDF, pdf);
+    std::cout << "Inlet fluid phase was changed (1 to fluid1; 2 to fluid2): " << c_.change_inlet_phase << std::endl;
 }
 
 void Simulation::write_info() const {
@@ -833,6 +882,7 @@ int Simulation::run(const std::string& job_status) {
         monitor_previous_ = -1.0;  // no convergence test on the first monitor after a restart
         monitor_current_ = -1.0;
     }
+    if (c_.change_inlet_phase == 1 || c_.change_inlet_phase == 2) change_inlet_fluid_phase();
     check(mflbm_color_gradient(h_), "color_gradient");
     {
         mflbm_monitor_out m{};

@@ -40,7 +40,7 @@ def test_valid_inputs_pass(tmp_path):
     ({"jper": 0}, "non-slip BC not applied at y"),
     ({"porous_plate_cmd": 3}, "porous_plate_cmd must be 0, 1 or 2"),
     ({"inlet_BC": 2, "outlet_BC": 1}, "Inlet pressure + outlet convective BC is not supported"),
-    ({"modify_geometry_cmd": 1}, "modify_geometry_cmd"),
+    ({"geometry_preprocess_cmd": 1}, "geometry_preprocessing_load is not implemented yet"),
 ])
 def test_rejected_controls(tmp_path, change, message):
     ctrl, geo = small_case()

@@ -210,3 +210,49 @@ def test_driver_porous_plate_run(tmp_path):
     s1 = (nx + 2) * (ny + 2) * (nz + 2)
     assert nt == 61 and np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
     ref.close()
+
+
+def test_driver_modified_geometry_and_inlet_phase_switch(tmp_path):
+    """modify_geometry_cmd = 1 on a duct (no geometry file) and change_inlet_fluid_phase_cmd = 2
+    on the resumed run, against the same edits made through the Python API."""
+    root = str(tmp_path)
+    n = (24, 24, 40)
+    nx, ny, nz = n
+    ctrl = cases.drainage_control(n=n, interface_z0=6.0, jper=0)
+    ctrl.update(max_time_step=59, monitor_timer=20, monitor_profile_timer_ratio=100, ntime_visual=1000,
+                ntime_animation=1000, computation_time_timer=20, display_steps_timer=20, n_exclude_inlet=2,
+                n_exclude_outlet=2, steady_state_option=0, checkpoint_save_timer=1e3, checkpoint_2rd_save_timer=1e3,
+                simulation_duration_timer=1e3, modify_geometry_cmd=1, external_geometry_read_cmd=0)
+    write_case(root, ctrl, np.zeros(n[::-1], np.int8))
+    say("driver: modified duct geometry, 60 steps")
+    out = run_driver(root)
+    assert "Internal geometry modified!" in out
+    # the sample edit of src/Misc.cpp:105-136: sphere r = ny/4 at the centre, tube r = ny/2
+    # outside 10-layer reservoirs
+    k, j, i = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    d2 = (i - 0.5 * (nx + 1)) ** 2 + (j - 0.5 * (ny + 1)) ** 2 + (k - 0.5 * (nz + 1)) ** 2
+    geo = ((d2 < (0.25 * ny) ** 2) | ((d2 > (0.5 * ny) ** 2) & (k > 10) & (k < nz - 10 + 1))).astype(np.int8)
+    p = config.derive(ctrl)
+    ref = gpu.Solver(p, geo, gpu.FP64)
+    ref.setup()
+    ref.run(1, 60)
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 61 and np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+
+    write_case(root, dict(ctrl, change_inlet_fluid_phase_cmd=2, max_time_step=39), np.zeros(n[::-1], np.int8),
+               status="continue_simulation")
+    say("driver: resume with change_inlet_fluid_phase_cmd = 2, 40 steps")
+    out = run_driver(root)
+    assert "Inlet fluid phase was changed (1 to fluid1; 2 to fluid2): 2" in out
+    pdf = ref.download("pdf")  # (2, 19, nz + 2, ny + 2, nx + 2)
+    below = np.arange(nz + 2) <= 6.0
+    sl = (slice(None), below, slice(None), slice(0, nx))  # i = 0 .. nx - 1 (src/Misc.cpp:283-285)
+    pdf[1][sl] += pdf[0][sl]
+    pdf[0][sl] = 0.0
+    ref.upload("pdf", pdf)
+    ref.color_gradient()
+    ref.run(61, 40)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 101 and np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    ref.close()
