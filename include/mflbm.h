@@ -198,6 +198,13 @@ int mflbm_ensemble_solve(mflbm_ensemble* e, int32_t check, double tol, int32_t m
  * minus the domain's rest density (rho_ref of mflbm_ensemble_create); zero at solid nodes. */
 int mflbm_ensemble_fields(mflbm_ensemble* e, int32_t first, int32_t count, float* out);
 
+/* Initial state of domains first .. first+count-1 from fields in the layout of
+ * mflbm_ensemble_fields (count x 4 x npt: u, v, w, rho - rho_ref): the relaxed equilibrium
+ * of mflbm_init_relaxed at those values.  Only before the domains' first step; the first
+ * steady test then compares with this state.  (No counterpart in the reference: the warm
+ * start of the ensemble.) */
+int mflbm_ensemble_init_fields(mflbm_ensemble* e, int32_t first, int32_t count, const float* fields);
+
 #ifdef __cplusplus
 }
 #endif

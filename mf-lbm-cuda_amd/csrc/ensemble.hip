@@ -160,6 +160,30 @@ __global__ __launch_bounds__(EB) void k_ens_scatter(EnsDev E, const float* __res
     o[3 * npt] = drho[t];
 }
 
+// Initial populations of the fluid nodes of domains first .. from dense fields (u, v, w and
+// rho - rho_d per node, the layout of mflbm_ensemble_fields): the collision's rest weights
+// times rho plus the reference's velocity terms, stored as deviations from rho_d r_q (the
+// FP64 arithmetic of k_init_eq's relaxed FP32 branch).
+__global__ __launch_bounds__(EB) void k_ens_init(EnsDev E, const double* __restrict__ rho_ref,
+                                                 const int32_t* __restrict__ loc, int64_t cell0, int64_t ncell,
+                                                 int first, int64_t npt, const float* __restrict__ in) {
+    const int64_t t = cell0 + (int64_t)blockIdx.x * EB + threadIdx.x;
+    if (t >= cell0 + ncell) return;
+    const int d = E.blk_dom[t / EB];
+    if (t >= (int64_t)E.dom_start[d] + E.dom_count[d]) return;
+    const float* x = in + (int64_t)(d - first) * 4 * npt + loc[t];
+    const double uu = x[0], vv = x[npt], ww = x[2 * npt], rd = rho_ref[d], rr = rd + (double)x[3 * npt];
+    const double usq = uu * uu + vv * vv + ww * ww;
+    const double cu[19] = {0.0, uu, -uu, vv, -vv, ww, -ww, uu + vv, -uu + vv, uu - vv, -uu - vv,
+                           uu + ww, -uu + ww, uu - ww, -uu - ww, vv + ww, -vv + ww, vv - ww, -vv - ww};
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const double wqq = q == 0 ? 1.0 / 3.0 : (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+        const double shape = wqq * (3.0 * cu[q] + 4.5 * cu[q] * cu[q] - 1.5 * usq);
+        E.f[q * E.ns + t] = (float)((rr - rd) * rest_weight(q) + rr * shape);
+    }
+}
+
 }  // namespace
 
 struct mflbm_ensemble {
@@ -180,6 +204,7 @@ struct mflbm_ensemble {
     // per-domain progress, kept across solve calls
     std::vector<int32_t> taken, checks;
     std::vector<double> perm, change;
+    std::vector<double> rho_d;  // rest density of each domain
     std::string err;
 };
 
@@ -308,7 +333,8 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
     e->nf = nf;
     e->ns = (nf + ext_max + 1023) / 1024 * 1024;
     if (e->ns >= INT32_MAX) return fail(e, "ensemble: too many cells for 32-bit cell indices");
-    if (rho_ref)
+    e->rho_d.assign(nd, 1.0);
+    {
         for (int d = 0; d < nd; ++d) {
             // the cold start's mass (D3Q19 weights in every bounce-back link) at the relaxed
             // weights (the summation order of relaxed_mean_density in mflbm/warmstart.py)
@@ -318,8 +344,10 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
                 w_std += links * (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
                 r_rel += links * (q <= 6 ? 13.0 / 126.0 : 1.0 / 63.0);
             }
-            rho_ref[d] = ((double)cnt[d].nf + w_std) / ((double)cnt[d].nf + r_rel);
+            e->rho_d[d] = ((double)cnt[d].nf + w_std) / ((double)cnt[d].nf + r_rel);
+            if (rho_ref) rho_ref[d] = e->rho_d[d];
         }
+    }
 
     // pass 2: neighbour rows, node coordinates and block owners
     std::vector<int32_t> nb((size_t)18 * nf, 0), loc((size_t)nf, 0), blk((size_t)(nf / EB), 0);
@@ -465,6 +493,40 @@ int mflbm_ensemble_solve(mflbm_ensemble* e, int32_t check, double tol, int32_t m
         if (change) change[d] = e->change[d];
     }
     if (device_ms) *device_ms = ms_total;
+    return 0;
+}
+
+int mflbm_ensemble_init_fields(mflbm_ensemble* e, int32_t first, int32_t count, const float* fields) {
+    if (!e || !e->f || !fields) return -1;
+    if (first < 0 || count <= 0 || first + count > e->nd) return fail(e, "ensemble: domain range out of bounds");
+    for (int d = first; d < first + count; ++d)
+        if (e->taken[d] > 0) return fail(e, "ensemble: initial fields only before a domain's first step");
+    const int64_t cell0 = e->start[first];
+    const int64_t cell1 = (int64_t)e->start[first + count - 1] + (e->count[first + count - 1] + EB - 1) / EB * EB;
+    const int64_t ncell = cell1 - cell0;
+    const size_t bytes = (size_t)count * 4 * e->npt * sizeof(float);
+    float* buf = nullptr;
+    double* rref = nullptr;
+    ECHK(e, hipMalloc(&buf, bytes));
+    hipError_t r = hipMalloc(&rref, (size_t)e->nd * sizeof(double));
+    if (r == hipSuccess) r = hipMemcpy(buf, fields, bytes, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemcpy(rref, e->rho_d.data(), (size_t)e->nd * sizeof(double), hipMemcpyHostToDevice);
+    const EnsDev E = dev_view(e);
+    if (r == hipSuccess && ncell > 0) {
+        k_ens_init<<<(unsigned)((ncell + EB - 1) / EB), EB, 0, e->stream>>>(E, rref, e->loc, cell0, ncell, first,
+                                                                           e->npt, buf);
+        r = hipGetLastError();
+    }
+    // the steady test of the first check compares with this state
+    if (r == hipSuccess) {
+        k_ens_check<<<(unsigned)(e->nf / EB), EB, 0, e->stream>>>(E, e->ux, e->uy, e->uz, e->drho, e->part);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    (void)hipFree(buf);
+    (void)hipFree(rref);
+    if (r != hipSuccess) return fail(e, "ensemble init: %s", hipGetErrorString(r));
+    for (int d = first; d < first + count; ++d) e->checks[d] = 1;
     return 0;
 }
 

@@ -76,6 +76,17 @@ class Ensemble:
                                                     out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
+    def init_fields(self, fields, first=0):
+        """Start domains first .. from fields (count, 4, nz, ny, nx): u, v, w and rho - rho_ref
+        (the layout of fields()); only before their first step."""
+        a = np.ascontiguousarray(fields, dtype=np.float32)
+        if a.ndim == 4:
+            a = a[None]
+        if a.shape[1:] != (4,) + self.shape:
+            raise ValueError(f"fields must be (count, 4) + {self.shape}")
+        self._check(gpu.load().mflbm_ensemble_init_fields(self.h, int(first), int(a.shape[0]),
+                                                         a.ctypes.data_as(C.POINTER(C.c_float))))
+
     def velocity(self, first=0, count=None) -> np.ndarray:
         return self.fields(first, count)[:, :3]
 

@@ -28,7 +28,8 @@ EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi"
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
            "mflbm_macro", "mflbm_monitor", "mflbm_velocity_change", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe",
            "mflbm_ensemble_create", "mflbm_ensemble_destroy", "mflbm_ensemble_last_error",
-           "mflbm_ensemble_fluid_nodes", "mflbm_ensemble_solve", "mflbm_ensemble_fields"]
+           "mflbm_ensemble_fluid_nodes", "mflbm_ensemble_solve", "mflbm_ensemble_fields",
+           "mflbm_ensemble_init_fields"]
 
 
 class Params(C.Structure):
@@ -109,6 +110,8 @@ def load():
     L.mflbm_ensemble_solve.restype = C.c_int
     L.mflbm_ensemble_fields.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
     L.mflbm_ensemble_fields.restype = C.c_int
+    L.mflbm_ensemble_init_fields.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+    L.mflbm_ensemble_init_fields.restype = C.c_int
     for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_init_relaxed", "mflbm_color_gradient",
                  "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro",
                  "mflbm_monitor", "mflbm_sync"):

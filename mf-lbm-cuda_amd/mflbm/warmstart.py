@@ -331,6 +331,34 @@ class WarmStart:
         return tuple(to_s1(a) for a in (rho, *vel))
 
 
+def ensemble_run(params, geo, fields=None, tol=5e-5, check=500, max_steps=400000, probe=None, info=None):
+    """One domain through the single-phase ensemble kernel (FP32 deviations), from rest or
+    from fields (4, nz, ny, nx) = u, v, w, rho - rho_ref; probe as in run_to_steady.
+    Returns (steps, permeability or None after a failed probe, wall seconds without set-up)."""
+    from . import ensemble
+    ens = ensemble.Ensemble(params, geo[None])
+    t0 = time.perf_counter()
+    taken = 0
+    if fields is not None:
+        ens.init_fields(fields)
+    if probe:
+        pre = probe[0] - PROBE_WINDOW
+        if pre > 0:
+            ens.solve(check=pre, tol=-1.0, max_steps=pre)
+        out = ens.solve(check=PROBE_WINDOW, tol=-1.0, max_steps=PROBE_WINDOW)
+        taken = probe[0]
+        residual = float(out["change"][0]) / PROBE_WINDOW
+        if info is not None:
+            info["residual"] = residual
+        if residual > probe[1]:
+            ens.close()
+            return taken, None, time.perf_counter() - t0
+    out = ens.solve(check=check, tol=tol, max_steps=max_steps - taken)
+    seconds = time.perf_counter() - t0
+    ens.close()
+    return abs(int(out["steps"][0])), float(out["permeability"][0]), seconds
+
+
 def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=500, tol=5e-5,
                   cold_max_steps=400000, fallback_step=500, fallback_residual=1e-4, starts=("cold", "relaxed", "warm")):
     """Held-out geometries twice as fine as the network's grid: the reference's FP64 cold
@@ -391,6 +419,31 @@ def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=
             log(f"held-out {s}: {name} {steps} steps {row[f'{name}_s']:.1f} s, k {row[f'k_{name}']:.5f}"
                 + (f", fallback {row['fallback']}" if name == "warm" else ""))
         sim.close()
+        # the same starts through the single-phase ensemble kernel (phi = 1 throughout)
+        for name in [n for n in ("relaxed_ens", "warm_ens") if n in starts]:
+            if name == "warm_ens":
+                t0 = time.perf_counter()
+                pr = [a[1:-1, 1:-1, 1:-1] for a in ws.predict_fine(geo, params)]
+                fields = np.stack([pr[1], pr[2], pr[3], np.where(fluid, pr[0] - relaxed_mean_density(geo), 0.0)])
+                t_pred = time.perf_counter() - t0
+                del pr
+                info = {}
+                steps, k, sec = ensemble_run(params, geo, fields, tol, check, cold_max_steps,
+                                             probe=(fallback_step, fallback_residual), info=info)
+                row["ens_residual_at_probe"] = info.get("residual")
+                row["ens_fallback"] = k is None
+                if k is None:
+                    lost, lost_s = steps, sec
+                    steps, k, sec = ensemble_run(params, geo, None, tol, check, cold_max_steps)
+                    steps, sec = steps + lost, sec + lost_s
+                sec += t_pred
+            else:
+                steps, k, sec = ensemble_run(params, geo, None, tol, check, cold_max_steps)
+            row[f"{name}_steps"], row[f"{name}_s"], row[f"k_{name}"] = steps, sec, k
+            if "k_cold" in row:
+                row[f"k_{name}_rel_diff"] = abs(k - row["k_cold"]) / abs(row["k_cold"])
+            log(f"held-out {s}: {name} {steps} steps {sec:.2f} s, k {k:.5f}"
+                + (f", fallback {row['ens_fallback']}" if name == "warm_ens" else ""))
         rows.append(row)
     return rows
 

@@ -128,3 +128,37 @@ def test_solve_in_chunks_equals_one_call():
     two.close()
     assert np.array_equal(a["steps"], b["steps"])
     assert np.array_equal(va, vb)
+
+
+def test_init_fields_starts_from_the_given_state():
+    """mflbm_ensemble_init_fields: the stored state is the relaxed equilibrium of the given
+    fields (checked through the moments the steady test reads), and a start from the steady
+    fields of a converged ensemble reaches the same state, in no more steps."""
+    n = (20, 16, 24)
+    p = params(n)
+    geos = geometries(n, [4, 7])
+    a = ensemble.Ensemble(p, geos)
+    out_a = a.solve(check=100, tol=1e-5, max_steps=20000)
+    fa = a.fields()
+    a.close()
+    b = ensemble.Ensemble(p, geos)
+    init = fa.copy()
+    # fields() holds the steady test's velocity, the momentum less F / 2 along z
+    init[:, 2] += np.where(fa[:, 2] != 0, 0.5 * p["force_z"], 0.0).astype(np.float32)
+    b.init_fields(init)
+    f0 = b.fields()
+    for d in range(2):
+        fluid = ~warmstart.solid_with_walls(geos[d])
+        rho = b.rho_ref[d] + fa[d, 3].astype(np.float64)
+        # x momentum = rho u_x (no force along x); density deviation exact
+        assert np.abs(f0[d, 0][fluid] - rho[fluid] * fa[d, 0][fluid]).max() < 1e-6 * np.abs(fa[d, 0]).max()
+        assert np.abs(f0[d, 3][fluid] - fa[d, 3][fluid]).max() < 1e-6
+    out_b = b.solve(check=100, tol=1e-5, max_steps=20000)
+    fb = b.fields()
+    b.close()
+    say(f"steps from rest {out_a['steps']}, from the steady fields {out_b['steps']}")
+    assert (out_b["steps"] > 0).all() and (out_b["steps"] <= out_a["steps"]).all()
+    for d in range(2):
+        err = np.linalg.norm(fb[d, :3] - fa[d, :3]) / np.linalg.norm(fa[d, :3])
+        assert err < 1e-4, err
+        assert abs(out_b["permeability"][d] / out_a["permeability"][d] - 1) < 1e-4
