@@ -427,3 +427,89 @@ def test_laplace_law():
     R_eff = (3 * vol / (4 * np.pi)) ** (1 / 3)
     dp = (rin - rout) / 3.0
     assert abs(dp - 2 * p["lbm_gamma"] / R_eff) < 0.15 * 2 * p["lbm_gamma"] / R_eff, (dp, 2 * p["lbm_gamma"] / R_eff)
+
+
+# ------------------------------------------------------------------------------------
+# K3 / K6: extrapolation of phi and of the normals to solid-boundary nodes
+# (src/main_iteration_GPU.cu:732-755, 880-906)
+# ------------------------------------------------------------------------------------
+def _shifted(a, e):
+    """a[x + e] for every x of the interior (one-node margin) of a padded array"""
+    ex, ey, ez = e
+    return a[1 + ez:a.shape[0] - 1 + ez, 1 + ey:a.shape[1] - 1 + ey, 1 + ex:a.shape[2] - 1 + ex]
+
+
+def _weighted_fluid_mean(values, wt, sel):
+    """sum over the 18 links of w_q * values(x + e_q) where wt(x + e_q) <= 0, over the sum of
+    those w_q, at the nodes sel (interior of the padded arrays)"""
+    num = np.zeros(sel.shape)
+    den = np.zeros(sel.shape)
+    for q in range(1, 19):
+        e = tuple(int(c) for c in TB.E[q])
+        f = _shifted(wt, e) <= 0
+        num += np.where(f, _shifted(values, e), 0.0) * TB.W[q]
+        den += np.where(f, TB.W[q], 0.0)
+    return num, den
+
+
+def test_phi_extrapolation_is_the_weighted_mean_of_fluid_neighbours():
+    """every solid-boundary node of [-2, n+3] holds the D3Q19-weighted mean of the phi of its
+    fluid neighbours (brute force over the whole array after the oracle's gradient chain)"""
+    geo, p = small_drainage(n=(16, 14, 20), seed=3, radius=3.0, porosity=0.55)
+    o = O.Oracle(p, geo)
+    o.setup()
+    rng = np.random.default_rng(5)
+    phi = o.array("phi")
+    phi[...] = rng.uniform(-1, 1, phi.shape)  # a field with no symmetry to hide a wrong weight
+    o.color_gradient()
+    wt = o.int_array("walls_type").astype(np.int64)
+    core = wt[1:-1, 1:-1, 1:-1]
+    num, den = _weighted_fluid_mean(phi, wt, core)
+    sb = (core == 2) & (den > 0)
+    assert sb.sum() > 500
+    assert np.abs(phi[1:-1, 1:-1, 1:-1][sb] - num[sb] / den[sb]).max() < 1e-14
+
+
+def test_normal_extrapolation_is_the_weighted_mean_of_fluid_neighbours():
+    """every solid-boundary node of [0, n+1] holds the D3Q19-weighted mean of the (wetting-
+    corrected) normals of its fluid neighbours, not renormalised"""
+    geo, p = small_drainage(n=(16, 14, 20), seed=3, radius=3.0, porosity=0.55)
+    o = O.Oracle(p, geo)
+    o.setup()
+    o.run(1, 30)  # an interface with normals near the solids
+    wt4 = o.int_array("walls_type").astype(np.int64)
+    wt = wt4[2:-2, 2:-2, 2:-2]  # s4 -> s2 footprint
+    for name in ("cn_x", "cn_y", "cn_z"):
+        cn = o.array(name)
+        num, den = _weighted_fluid_mean(cn, wt, wt[1:-1, 1:-1, 1:-1])
+        sb = np.zeros(wt.shape, bool)
+        sb[2:-2, 2:-2, 2:-2] = wt[2:-2, 2:-2, 2:-2] == 2  # [0, n+1]
+        sb = sb[1:-1, 1:-1, 1:-1] & (den > 0)
+        assert sb.sum() > 500
+        assert np.abs(cn[1:-1, 1:-1, 1:-1][sb] - num[sb] / den[sb]).max() < 1e-14, name
+    assert np.abs(o.array("cn_z")).max() > 0.5
+
+
+def test_extrapolation_at_a_flat_wall_takes_the_first_fluid_plane():
+    """closed form: at a flat wall every fluid neighbour of a wall node lies in the first fluid
+    plane, so the extrapolated phi (and normal) there is that plane's value when the field
+    does not vary along the wall"""
+    p = base_params(kper=0, wall_z_min=1, wall_z_max=1, inlet_bc=0, outlet_bc=0, force_z=0.0)
+    geo = np.zeros((8, 8, 8), np.int8)
+    o = O.Oracle(p, geo)
+    o.setup(option=1, interface_z0=4.0)
+    phi = o.array("phi")
+    zs = np.arange(phi.shape[0], dtype=float)
+    phi[...] = np.tanh((zs - 7.5) / 2.0)[:, None, None]  # varies along z only
+    o.color_gradient()
+    wt = o.int_array("walls_type")
+    # the z-min wall layer k = 1 (s4 index 4): solid-boundary in the interior columns; its
+    # fluid neighbours are in the plane k = 2 (s4 index 5)
+    col = (slice(None), slice(5, 9), slice(5, 9))
+    assert (wt[4][col[1:]] == 2).all() and (wt[5][col[1:]] <= 0).all()
+    assert np.abs(phi[4][col[1:]] - phi[5][col[1:]]).max() < 1e-15
+    # normals: columns i = 4, 5, whose fluid neighbours are clear of the x walls (next to those
+    # the wetting correction tilts the first plane's normals)
+    cz = o.array("cn_z")  # s2: k = 1 at index 2, k = 2 at index 3
+    a, b = cz[2][2:10, 5:7], cz[3][2:10, 5:7]
+    assert np.abs(a - b).max() < 1e-15 and np.abs(b).min() > 0.1
