@@ -75,8 +75,6 @@ struct mflbm_handle {
 
 namespace {
 
-constexpr int64_t SLOT_SKEW_DEFAULT = 0;
-
 size_t esize(const mflbm_handle* h) { return h->prec == MFLBM_FP64 ? sizeof(double) : sizeof(float); }
 
 int fail(mflbm_handle* h, const char* fmt, ...) {
@@ -518,15 +516,9 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
     std::string lerr;
     if (!build_layout(d, lf, (int)p.kper, (int)p.jper, h->open_z, h->in_bc, h->out_bc, h->plate_z, lay, lerr))
         return fail(h, "layout: %s", lerr.c_str());
-    // slot-array stride: the cells a step uses (fluid nodes + extra cells), rounded up to a
-    // 4 KiB multiple plus a skew, so that the 38 streams of one node do not start at the
-    // same offset within the HBM channel interleave; MFLBM_SLOT_SKEW (elements) overrides
-    {
-        int64_t skew = SLOT_SKEW_DEFAULT;
-        if (const char* env = std::getenv("MFLBM_SLOT_SKEW")) skew = std::max(0, std::atoi(env));
-        const int64_t q = 4096 / (int64_t)es;
-        h->ns = (lay.ns + q - 1) / q * q + skew;
-    }
+    // cells a step uses (fluid nodes + extra cells), in whole blocks of Sparse::CELL_BLOCK
+    constexpr int64_t cb = Sparse<double>::CELL_BLOCK;
+    h->ns = (lay.ns + cb - 1) / cb * cb;
     h->n_ext = lay.ext_off[19];
     HIPCHK(h, hipMalloc(&h->cells, 38 * (size_t)h->ns * es));
     HIPCHK(h, hipMemsetAsync(h->cells, 0, 38 * (size_t)h->ns * es, h->stream));

@@ -240,7 +240,6 @@ template <typename T, bool ODD>
 __global__ __launch_bounds__(256) void k_collide(Sparse<T> S, T* __restrict__ phi, Dims d, StepConst<T> P) {
     const int t = xcd_block() * 256 + threadIdx.x;
     if (t >= S.nf) return;
-    const int64_t NS = S.ns;
     T g1[19], g2[19], o1[19], o2[19];
     int nb[19];
     if (ODD) {
@@ -252,8 +251,8 @@ __global__ __launch_bounds__(256) void k_collide(Sparse<T> S, T* __restrict__ ph
     for (int q = 0; q < 19; ++q) {
         const int64_t src = ODD ? (int64_t)nb[OPP[q]] : (int64_t)t;
         const int slot = ODD ? q : OPP[q];
-        g1[q] = S.f[slot * NS + src];
-        g2[q] = S.f[(slot + 19) * NS + src];
+        g1[q] = S.f[S.at(slot, src)];
+        g2[q] = S.f[S.at(slot + 19, src)];
     }
     const T* a = S.aux + 4 * (int64_t)t;
     const T ph = collide_node<T>(P, g1, g2, a[0], a[1], a[2], a[3], o1, o2);
@@ -264,8 +263,8 @@ __global__ __launch_bounds__(256) void k_collide(Sparse<T> S, T* __restrict__ ph
     for (int q = 0; q < 19; ++q) {
         const int64_t dst = ODD ? (int64_t)nb[q] : (int64_t)t;
         const int slot = ODD ? OPP[q] : q;
-        S.f[slot * NS + dst] = o1[q];
-        S.f[(slot + 19) * NS + dst] = o2[q];
+        S.f[S.at(slot, dst)] = o1[q];
+        S.f[S.at(slot + 19, dst)] = o2[q];
     }
 }
 
@@ -277,9 +276,8 @@ __global__ __launch_bounds__(256) void k_periodic_pdf(Sparse<T> S, const int32_t
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
     const int slot = pairs[3 * t], dst = pairs[3 * t + 1], src = pairs[3 * t + 2];
-    const int64_t NS = S.ns;
-    S.f[slot * NS + dst] = S.f[slot * NS + src];
-    S.f[(slot + 19) * NS + dst] = S.f[(slot + 19) * NS + src];
+    S.f[S.at(slot, dst)] = S.f[S.at(slot, src)];
+    S.f[S.at(slot + 19, dst)] = S.f[S.at(slot + 19, src)];
 }
 
 // Periodic ghost layers of phi (periodic_phi_{Z,Y,ZY}, src/main_iteration_GPU.cu:1691-1733);
@@ -349,15 +347,14 @@ struct ColourRef {
 // maps (a, b, c, e) to the cell index.
 template <typename T, class CellOf, class Fn>
 __device__ __forceinline__ void with_cells(const Sparse<T>& S, T rho_ref, CellOf&& cell_of, Fn&& fn) {
-    const int64_t NS = S.ns;
     if constexpr (ShiftedStorage<T>::value) {
         auto f = [&](int a, int b, int c, int e, int g) -> ColourRef<T> {
             const int64_t cell = cell_of(a, b, c, e);
-            return ColourRef<T>{S.f + e * NS + cell, S.f + (e + 19) * NS + cell, rho_ref * (T)rest_weight(e), g == 1};
+            return ColourRef<T>{S.f + S.at(e, cell), S.f + S.at(e + 19, cell), rho_ref * (T)rest_weight(e), g == 1};
         };
         fn(f);
     } else {
-        auto f = [&](int a, int b, int c, int e, int g) -> T& { return S.f[(e + 19 * g) * NS + cell_of(a, b, c, e)]; };
+        auto f = [&](int a, int b, int c, int e, int g) -> T& { return S.f[S.at(e + 19 * g, cell_of(a, b, c, e))]; };
         fn(f);
     }
 }
@@ -435,8 +432,8 @@ __global__ __launch_bounds__(256) void k_fluid_cells(Sparse<T> S, T* __restrict_
     unpack_ijk(S.list[t], i, j, k);
     const int64_t c = i1(d, i, j, k);
     for (int s = 0; s < 38; ++s) {
-        if (TO_DENSE) dense[s * d.S1 + c] = S.f[s * S.ns + t];
-        else S.f[s * S.ns + t] = dense[s * d.S1 + c];
+        if (TO_DENSE) dense[s * d.S1 + c] = S.f[S.at(s, t)];
+        else S.f[S.at(s, t)] = dense[s * d.S1 + c];
     }
 }
 
@@ -452,8 +449,8 @@ __global__ __launch_bounds__(256) void k_extra_cells(Sparse<T> S, T* __restrict_
     const int64_t c = ext_node[t];
     for (int g = 0; g < 2; ++g) {
         const int sl = s + 19 * g;
-        if (TO_DENSE) dense[sl * d.S1 + c] = S.f[sl * S.ns + S.nf + e];
-        else S.f[sl * S.ns + S.nf + e] = dense[sl * d.S1 + c];
+        if (TO_DENSE) dense[sl * d.S1 + c] = S.f[S.at(sl, S.nf + e)];
+        else S.f[S.at(sl, S.nf + e)] = dense[sl * d.S1 + c];
     }
 }
 
@@ -750,7 +747,7 @@ __device__ __forceinline__ void macro_node(const Sparse<T>& S, const StepConst<T
     constexpr bool SH = ShiftedStorage<T>::value;
     T ft[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) ft[q] = SH ? S.f[(q + 19) * S.ns + t] : S.f[q * S.ns + t] + S.f[(q + 19) * S.ns + t];
+    for (int q = 0; q < 19; ++q) ft[q] = SH ? S.f[S.at(q + 19, t)] : S.f[S.at(q, t)] + S.f[S.at(q + 19, t)];
     T r = ft[0];
 #pragma unroll
     for (int q = 1; q < 19; ++q) r = r + ft[q];

@@ -106,12 +106,20 @@ struct ShiftedStorage<float> { static constexpr bool value = true; };
 // Fluid-compact population storage (see kernels.inl for the cell layout).
 template <typename T>
 struct Sparse {
-    T* f;                           // 38 * ns, slot (s + 19 g) at offset (s + 19 g) * ns (ShiftedStorage)
-    int64_t ns;                     // cells per slot array
+    T* f;                           // 38 * ns: slot (s + 19 g) of every cell (ShiftedStorage), at at()
+    int64_t ns;                     // cells (a multiple of CELL_BLOCK)
     int nf;                         // interior fluid nodes
     const int32_t* __restrict__ nb; // 18 * nf
     const uint32_t* __restrict__ list;  // nf packed coordinates
     T* aux;                         // 4 * nf force records (cn_x, cn_y, cn_z, 0.5 gamma curv c_norm)
+    // Position of slot s (0 .. 37) of cell c in f.  Cells come in blocks of CELL_BLOCK, each
+    // block holding its 38 slots one after the other: a wave of consecutive cells reads
+    // and writes 38 runs of 1 KiB (FP64) within one 38 KiB region rather than 38 separate
+    // arrays (measured 5 % faster for the even collide than one array per slot).
+    static constexpr int CELL_SHIFT = 7, CELL_BLOCK = 1 << CELL_SHIFT;
+    MFLBM_HD int64_t at(int s, int64_t c) const {
+        return ((c >> CELL_SHIFT) * 38 + s) * CELL_BLOCK + (c & (CELL_BLOCK - 1));
+    }
 };
 
 // Inputs of the fused gradient-chain kernel (gradient.inl).  `tag` holds, for every node of
