@@ -223,13 +223,13 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 
 // ---------------------------------------------------------------------------------------
 // Fluid-compact population storage.
-// The 38 slot arrays hold, per slot, the interior fluid nodes first (index = position in
-// the fluid list) followed by the slot's extra cells: every non-fluid (node, slot) pair a
-// step ever reads or writes (bounce-back buffers in solid neighbours, ghost and periodic
-// image cells, cells read by the Zou-He boundaries).  nb[(q-1)*nf + t] is the index of
-// cell (x_t + e_q, slot opp(q)) in slot array opp(q): an odd step pushes direction q into
-// that cell and pulls direction opp(q) out of it (the same cell, hence in place).  Fluid
-// neighbours have the same index in every slot array (their list position).
+// Cells hold, per slot, the interior fluid nodes first (index = position in the fluid list)
+// followed by the slot's extra cells: every non-fluid (node, slot) pair a step ever reads or
+// writes (bounce-back buffers in solid neighbours, ghost and periodic image cells, cells read
+// by the open boundaries); S.at(slot, cell) places them.  nb[(q-1)*nf + t] is the cell of
+// (x_t + e_q, slot opp(q)): an odd step pushes direction q into that cell and pulls
+// direction opp(q) out of it (the same cell, hence in place).  Fluid neighbours have the
+// same cell in every slot (their list position).
 // ---------------------------------------------------------------------------------------
 
 // AA-pattern collide/stream of one interior fluid node (src/main_iteration_GPU.cu:56-388
