@@ -496,7 +496,9 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
                 }
         HIPCHK(h, upload_i32_raw(&h->tag, tag));
         h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
-        h->grad_shape = GRAD_SHAPE_DEFAULT;
+        // FP32 runs 6 waves per workgroup (3 per SIMD): 5 % faster there; FP64 keeps 4 waves
+        // (6 measured 60 % slower)
+        h->grad_shape = p.precision == MFLBM_FP32 ? GRAD_SHAPE_FP32 : GRAD_SHAPE_DEFAULT;
         if (const char* env = std::getenv("MFLBM_GRAD_SHAPE")) h->grad_shape = std::atoi(env);
         if (h->grad_shape < 0 || h->grad_shape >= GRAD_NSHAPES) return fail(h, "MFLBM_GRAD_SHAPE out of range");
         const GradShape gs = GRAD_SHAPES[h->grad_shape];
