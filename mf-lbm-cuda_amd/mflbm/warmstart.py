@@ -359,6 +359,26 @@ def ensemble_run(params, geo, fields=None, tol=5e-5, check=500, max_steps=400000
     return abs(int(out["steps"][0])), float(out["permeability"][0]), seconds
 
 
+def coarse_start(geo, n_coarse, seed, tol=5e-5, check=500):
+    """Fields (4, nz, ny, nx) for ensemble_run from the steady state of the geometry coarsened
+    by two, solved from rest by the ensemble kernel and interpolated back with the same
+    length scaling as the network's prediction (velocity x4, density perturbation x2)."""
+    from scipy import ndimage
+    from . import ensemble
+    gc = coarsen(geo)
+    _, pc, _ = single_phase_case(seed, n_coarse)
+    ens = ensemble.Ensemble(pc, gc[None])
+    ens.solve(check=check, tol=tol, max_steps=400000)
+    fc = ens.fields()[0].astype(np.float64)
+    ens.close()
+    up = lambda a: ndimage.zoom(a, 2, order=1, grid_mode=True, mode="nearest")
+    fluid = ~solid_with_walls(geo)
+    u = [np.where(fluid, up(fc[c]) * 4.0, 0.0) for c in range(3)]
+    d = up(fc[3]) * 2.0
+    d = np.where(fluid, d - d[fluid].mean(), 0.0)
+    return np.stack(u + [d]).astype(np.float32)
+
+
 def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=500, tol=5e-5,
                   cold_max_steps=400000, fallback_step=500, fallback_residual=1e-4, starts=("cold", "relaxed", "warm")):
     """Held-out geometries twice as fine as the network's grid: the reference's FP64 cold
@@ -419,9 +439,25 @@ def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=
             log(f"held-out {s}: {name} {steps} steps {row[f'{name}_s']:.1f} s, k {row[f'k_{name}']:.5f}"
                 + (f", fallback {row['fallback']}" if name == "warm" else ""))
         sim.close()
-        # the same starts through the single-phase ensemble kernel (phi = 1 throughout)
-        for name in [n for n in ("relaxed_ens", "warm_ens") if n in starts]:
-            if name == "warm_ens":
+        # the same starts through the single-phase ensemble kernel (phi = 1 throughout), and
+        # a start from the steady state of the coarse geometry (solved by the same kernel)
+        for name in [n for n in ("relaxed_ens", "warm_ens", "coarse_ens") if n in starts]:
+            if name == "coarse_ens":
+                t0 = time.perf_counter()
+                fields = coarse_start(geo, n_coarse, s)
+                t_pre = time.perf_counter() - t0
+                info = {}
+                steps, k, sec = ensemble_run(params, geo, fields, tol, check, cold_max_steps,
+                                             probe=(fallback_step, fallback_residual), info=info)
+                row["coarse_residual_at_probe"] = info.get("residual")
+                row["coarse_fallback"] = k is None
+                if k is None:
+                    lost, lost_s = steps, sec
+                    steps, k, sec = ensemble_run(params, geo, None, tol, check, cold_max_steps)
+                    steps, sec = steps + lost, sec + lost_s
+                row["coarse_prepare_s"] = t_pre
+                sec += t_pre
+            elif name == "warm_ens":
                 t0 = time.perf_counter()
                 pr = [a[1:-1, 1:-1, 1:-1] for a in ws.predict_fine(geo, params)]
                 fields = np.stack([pr[1], pr[2], pr[3], np.where(fluid, pr[0] - relaxed_mean_density(geo), 0.0)])
@@ -443,7 +479,8 @@ def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=
             if "k_cold" in row:
                 row[f"k_{name}_rel_diff"] = abs(k - row["k_cold"]) / abs(row["k_cold"])
             log(f"held-out {s}: {name} {steps} steps {sec:.2f} s, k {k:.5f}"
-                + (f", fallback {row['ens_fallback']}" if name == "warm_ens" else ""))
+                + (f", fallback {row['ens_fallback']}" if name == "warm_ens" else "")
+                + (f", fallback {row['coarse_fallback']}" if name == "coarse_ens" else ""))
         rows.append(row)
     return rows
 
