@@ -803,9 +803,9 @@ int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out) {
 int mflbm_velocity_change(mflbm_handle* h, double* change) {
     if (!h || !change) return -1;
     const int64_t nb = (h->n_fluid + 255) / 256;
-    if (!h->vprev) {
-        HIPCHK(h, hipMalloc(&h->vprev, (size_t)(3 * h->n_fluid + 1) * sizeof(double)));
-        HIPCHK(h, hipMalloc(&h->vpart, (size_t)(2 * nb + 2) * sizeof(double)));
+    if (!h->vprev || !h->vpart) {  // first call (or a first call whose allocation failed)
+        if (!h->vprev) HIPCHK(h, hipMalloc(&h->vprev, (size_t)(3 * h->n_fluid + 1) * sizeof(double)));
+        if (!h->vpart) HIPCHK(h, hipMalloc(&h->vpart, (size_t)(2 * nb + 2) * sizeof(double)));
         h->vprev_valid = false;
     }
     double* out = h->vpart + 2 * nb;  // the two totals follow the block sums
