@@ -143,7 +143,7 @@ struct GradArgs {
 struct GradShape {
     int tx, ty, threads;
 };
-constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}, {16, 16, 384}};
+constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}, {16, 16, 384}, {32, 16, 512}};
 constexpr int GRAD_NSHAPES = sizeof(GRAD_SHAPES) / sizeof(GRAD_SHAPES[0]);
 constexpr int GRAD_SHAPE_DEFAULT = 0, GRAD_SHAPE_FP32 = 3;
 

@@ -79,7 +79,7 @@ template <typename T>
 void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_t s) {
     const int chunks = (A.d.nz + A.kchunk - 1) / A.kchunk;
     const unsigned grid = (unsigned)(A.tiles_x * A.tiles_y * chunks);
-    static_assert(GRAD_NSHAPES == 4, "one case per shape");
+    static_assert(GRAD_NSHAPES == 5, "one case per shape");
     switch (A.shape) {
 #define MFLBM_GRAD_SHAPE(n)                                                                                  \
     case n:                                                                                                  \
@@ -90,6 +90,7 @@ void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_
         MFLBM_GRAD_SHAPE(1)
         MFLBM_GRAD_SHAPE(2)
         MFLBM_GRAD_SHAPE(3)
+        MFLBM_GRAD_SHAPE(4)
 #undef MFLBM_GRAD_SHAPE
         default: break;
     }
