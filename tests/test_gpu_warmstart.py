@@ -40,3 +40,18 @@ def test_learned_warm_start_32():
     # on these geometries the network's fields do not slow the relaxed start down by more
     # than one check interval per case on average
     assert steps["warm"] <= steps["relaxed"] + 100 * len(rows)
+
+
+def test_fine_grid_starts_reach_the_cold_permeability():
+    """The fine-grid path (network on the geometry coarsened by two, the fallback rule, the
+    ensemble-kernel starts and the coarse-solve start) on a 32^3 refinement of a 16^3 pack:
+    every start reaches the FP64 cold start's permeability."""
+    ws = warmstart.WarmStart(seed=0)  # untrained: small fields, exercised for the plumbing
+    rows = warmstart.evaluate_fine(ws, [1000], n_coarse=16, log=say, check=100, tol=1e-5, cold_max_steps=60000,
+                                   starts=("cold", "relaxed", "warm", "relaxed_ens", "warm_ens", "coarse_ens"))
+    r = rows[0]
+    say(str({k: v for k, v in r.items() if k.endswith("steps") or "fallback" in k}))
+    for name in ("relaxed", "warm", "relaxed_ens", "warm_ens", "coarse_ens"):
+        assert r[f"{name}_steps"] > 0
+        assert r[f"k_{name}_rel_diff"] < 1e-3, (name, r[f"k_{name}_rel_diff"])
+    assert isinstance(r["fallback"], bool) and isinstance(r["coarse_fallback"], bool)
