@@ -250,6 +250,31 @@ def test_solid_normals_flat_wall_and_sphere():
     assert np.all(cosang > np.cos(np.radians(12)))
 
 
+def test_solid_normals_of_a_45_degree_wall():
+    """a periodic slab of solid between two planes j + k = const: the smoothed indicator is a
+    function of j + k alone, so the solid normals at the fluid-boundary nodes lie exactly
+    along +-(0, 1, 1) (a stencil offset or weight wrong on one axis breaks it), pointing into
+    the solid, of length |g| / (|g| + eps) (src/Geometry_preprocessing.cpp:377)"""
+    n = (32, 16, 16)
+    p = config.derive(cases.periodic_control(n=n))
+    k, j, i = np.meshgrid(np.arange(1, 17), np.arange(1, 17), np.arange(1, 33), indexing="ij")
+    geo = (((j + k) % 16 >= 6) & ((j + k) % 16 <= 9)).astype(np.int8)
+    o = O.Oracle(p, geo)
+    wt = o.int_array("walls_type")
+    sn = np.stack([o.array(c) for c in ("s_nx", "s_ny", "s_nz")])
+    xs = np.s_[12 + 3:21 + 3]  # x in [12, 21], beyond the reach of the x walls
+    sel = np.zeros(wt.shape, bool)
+    sel[4:-4, 4:-4, xs] = wt[4:-4, 4:-4, xs] == -1
+    assert sel.sum() > 300
+    assert np.abs(sn[0][sel]).max() < 1e-15
+    assert np.abs(sn[1][sel] - sn[2][sel]).max() < 1e-12
+    assert np.abs(np.hypot(sn[1][sel], sn[2][sel]) - 1.0).max() < 1e-5
+    kk, jj, _ = np.nonzero(sel)
+    m = (kk - 3 + jj - 3) % 16  # (j + k) mod 16 of each selected node (s4 index = coordinate + 3)
+    assert (m < 6).any() and (m > 9).any()
+    assert (sn[1][sel][m < 6] > 0).all() and (sn[1][sel][m > 9] < 0).all()
+
+
 # ------------------------------------------------------------------------------------
 # Zou-He boundaries: imposed densities and zero transverse momentum
 # ------------------------------------------------------------------------------------
