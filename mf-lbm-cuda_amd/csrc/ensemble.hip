@@ -62,7 +62,8 @@ __global__ __launch_bounds__(EB) void k_ens_collide(EnsDev E) {
     for (int q = 0; q < 19; ++q) {
         const int64_t src = ODD ? nb[OPP[q]] : t;
         const int slot = ODD ? q : OPP[q];
-        ft[q] = E.f[slot * NS + src];
+        // even: own cells, streamed once -> nontemporal (as k_collide)
+        ft[q] = ODD ? E.f[slot * NS + src] : __builtin_nontemporal_load(E.f + slot * NS + src);
     }
     float m_rho = ft[0];
 #pragma unroll
@@ -75,7 +76,8 @@ __global__ __launch_bounds__(EB) void k_ens_collide(EnsDev E) {
     for (int q = 0; q < 19; ++q) {
         const int64_t dst = ODD ? nb[q] : t;
         const int slot = ODD ? OPP[q] : q;
-        E.f[slot * NS + dst] = f[q];
+        if (ODD) E.f[slot * NS + dst] = f[q];
+        else __builtin_nontemporal_store(f[q], E.f + slot * NS + dst);
     }
 }
 
