@@ -56,7 +56,7 @@ __global__ __launch_bounds__(EB) void k_ens_collide(EnsDev E) {
     if (ODD) {
         nb[0] = t;
 #pragma unroll
-        for (int q = 1; q < 19; ++q) nb[q] = E.nb[(int64_t)(q - 1) * NF + t];
+        for (int q = 1; q < 19; ++q) nb[q] = __builtin_nontemporal_load(E.nb + (int64_t)(q - 1) * NF + t);
     }
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
