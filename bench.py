@@ -57,7 +57,7 @@ def cpu_model():
 def cpu_baseline(geo, params, steps):
     """Time the OpenMP FP64 oracle on this host over `steps` steps of the same workload."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle as O  # the checker doubles as the reported CPU baseline
+    import oracle as O  # the oracle doubles as the reported CPU baseline
 
     threads = O.threads()
     o = O.Oracle(params, geo)
@@ -96,7 +96,7 @@ def pmc_child(args):
     prec = gpu.FP64 if args.precision == "fp64" else gpu.FP32
     geo = np.load(args.pmc_child, allow_pickle=False)
     n = geo.shape[0]
-    _, ctrl = cases.config2(n=(n, n, n), geometry=False)
+    _, ctrl = cases.drainage256(n=(n, n, n), geometry=False)
     sim = gpu.Solver(config.derive(ctrl), geo, prec)
     sim.setup()
     for t in range(1, PMC_STEPS + 1):
@@ -219,7 +219,7 @@ def main():
     s = 8 if prec == gpu.FP64 else 4
     n = args.n
     log(f"generating {n}^3 sphere pack")
-    geo, ctrl = cases.config2(n=(n, n, n))
+    geo, ctrl = cases.drainage256(n=(n, n, n))
     params = config.derive(ctrl)
 
     traffic = None

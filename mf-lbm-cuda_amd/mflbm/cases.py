@@ -65,7 +65,7 @@ def sphere_geometry(n, radius, porosity, seed, jper=1, buffer=0):
     return geo
 
 
-def config1(seed=1):
+def drainage64(seed=1):
     """64^3 FP64 drainage: spheres r=6, porosity 0.40, theta 45, tau 0.7, gamma 0.01."""
     n = (64, 64, 64)
     ctrl = drainage_control(n=n, theta=45.0, gamma=0.01, body_force=5e-4, interface_z0=4.0)
@@ -73,7 +73,7 @@ def config1(seed=1):
     return geo, ctrl
 
 
-def config2(seed=2, n=(256, 256, 256), geometry=True):
+def drainage256(seed=2, n=(256, 256, 256), geometry=True):
     """256^3 drainage: spheres r=8, porosity 0.35, theta 30, viscosity ratio 0.1.
     The invading fluid 1 has tau 0.7 and the defending fluid 2 ten times its viscosity
     (tau 2.5); scaling fluid 1 down to a tenth instead (tau 0.52) is unstable and the run
@@ -87,7 +87,7 @@ def config2(seed=2, n=(256, 256, 256), geometry=True):
     return geo, ctrl
 
 
-def config3(seed=3, n=(256, 256, 512), geometry=True):
+def grf_drainage512(seed=3, n=(256, 256, 512), geometry=True):
     """Precision-study drainage, 512 nodes along the flow axis and 256 x 256 across: seeded
     Gaussian-random-field medium (correlation length 10, porosity 0.30) with 8 empty layers
     at the inlet and outlet; the physics of the 256^3 workload (theta 30, viscosity ratio

@@ -122,7 +122,7 @@ def sphere_pack_ensemble(seeds, n=64, workers=8):
 
 
 def main(argv=None):
-    """Config 4: n domains of 64^3, porosity U[0.30, 0.50], radius U[4, 10], body force 1e-6,
+    """Ensemble of n single-phase domains of 64^3, porosity U[0.30, 0.50], radius U[4, 10], body force 1e-6,
     each to a relative velocity change below 1e-7 per step (tol per `check` steps)."""
     import argparse
     import json

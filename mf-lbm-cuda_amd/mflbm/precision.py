@@ -12,7 +12,7 @@ quantities the reference derives (src/Monitor.cpp:17-273), all reduced in FP64 o
 The phase fluxes under the same driving force stand for the relative permeabilities
 (k_i = nu_i u_i / grad p), so their relative errors are the relative-permeability errors.
 
-    python -m mflbm.precision --case config3 --steps 50000 --interval 1000
+    python -m mflbm.precision --case grf_drainage512 --steps 50000 --interval 1000
 """
 from __future__ import annotations
 
@@ -42,8 +42,8 @@ def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-def run(case="config3", steps=50000, interval=1000, n=None, log=print) -> dict:
-    make = {"config2": cases.config2, "config3": cases.config3}[case]
+def run(case="grf_drainage512", steps=50000, interval=1000, n=None, log=print) -> dict:
+    make = {"drainage256": cases.drainage256, "grf_drainage512": cases.grf_drainage512}[case]
     geo, ctrl = make(n=n) if n is not None else make()
     params = config.derive(ctrl)
     log(f"{case}: lattice {params['nx']}x{params['ny']}x{params['nz']}, porosity {1 - geo.mean():.3f}")
@@ -92,7 +92,7 @@ def run(case="config3", steps=50000, interval=1000, n=None, log=print) -> dict:
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--case", choices=["config2", "config3"], default="config3")
+    ap.add_argument("--case", choices=["drainage256", "grf_drainage512"], default="grf_drainage512")
     ap.add_argument("--steps", type=int, default=50000)
     ap.add_argument("--interval", type=int, default=1000)
     ap.add_argument("--out", default=None, help="write the JSON result here as well")

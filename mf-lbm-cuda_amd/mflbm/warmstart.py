@@ -21,7 +21,7 @@ builds the populations from (rho, u, v, w, phi) with the D3Q19 weights 1/3, 1/18
    first step are pre-collision values whose momentum is rho u - F / 2, so F / 2 is taken
    off the predicted velocity.
 
-The solver stays the judge: every start runs to the same steady criterion.  `evaluate`
+The cold-start solver stays the arbiter: every start runs to the same steady criterion.  `evaluate`
 reports, per held-out geometry, the reference's cold start (FP64), the relaxed start from
 rest (the method without the network) and the learned warm start, the last two in the
 chosen precision (FP32 by default).
@@ -382,7 +382,7 @@ def coarse_start(geo, n_coarse, seed, tol=5e-5, check=500):
 def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=500, tol=5e-5,
                   cold_max_steps=400000, fallback_step=500, fallback_residual=1e-4, starts=("cold", "relaxed", "warm")):
     """Held-out geometries twice as fine as the network's grid: the reference's FP64 cold
-    start (the judge), the relaxed start from rest and the learned warm start in `precision`,
+    start (the arbiter), the relaxed start from rest and the learned warm start in `precision`,
     with the fallback rule on the warm start."""
     rows = []
     for s in seeds:
@@ -486,7 +486,7 @@ def evaluate_fine(ws, seeds, n_coarse=128, precision=gpu.FP32, log=print, check=
 
 
 def evaluate(ws, seeds, n=64, precision=gpu.FP32, log=print):
-    """Per held-out geometry: the reference's cold start in FP64 (the judge), the relaxed
+    """Per held-out geometry: the reference's cold start in FP64 (the arbiter), the relaxed
     start from rest and the learned warm start, both in `precision`."""
     rows = []
     for s in seeds:

@@ -2,7 +2,7 @@
 
 FP64 handles must agree bit for bit on every field the reference keeps (the 38
 population slots including ghosts, phi, normals, curvature at fluid nodes) — the
-north-star bound is a relative L-infinity of 1e-12, bit equality is stronger.  Monitor
+required bound is a relative L-infinity of 1e-12, bit equality is stronger.  Monitor
 sums are reduced in a different order on the device; they must agree to 1e-12 of the
 summed magnitudes.
 """
@@ -184,9 +184,9 @@ def test_fused_gradient_fp32_matches_split():
 
 def test_config1_1000_steps():
     """64^3 drainage of the first workload, 1000 steps, relative L-inf <= 1e-12 (bitwise expected)."""
-    geo, ctrl = cases.config1()
+    geo, ctrl = cases.drainage64()
     p = config.derive(ctrl)
-    say("case=config1 64^3 fp64 1000 steps")
+    say("case=drainage64 64^3 fp64 1000 steps")
     o, g = pair(geo, p)
     o.setup()
     g.setup()
@@ -195,7 +195,7 @@ def test_config1_1000_steps():
     for n in ("pdf", "phi"):
         r = rel_linf(g.download(n), o.array(n))
         assert r <= 1e-12, f"{n}: relative Linf {r:.3e}"
-    compare_state(o, g, tag="config1 step1000")
+    compare_state(o, g, tag="drainage64 step1000")
     rho_o, u_o, v_o, w_o = o.macro(zero_solid_phi=False)
     rho_g, u_g, v_g, w_g = g.macro()
     for a, b, n in ((rho_g, rho_o, "rho"), (u_g, u_o, "u"), (v_g, v_o, "v"), (w_g, w_o, "w")):

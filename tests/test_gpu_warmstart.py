@@ -1,6 +1,6 @@
 """The learned warm start end to end on the GPU: training fields from the batched ensemble,
 the U-Net trained by the project's kernels, and held-out geometries solved three ways --
-the reference's cold start in FP64 (the judge), the relaxed start from rest in FP32 (the
+the reference's cold start in FP64 (the arbiter), the relaxed start from rest in FP32 (the
 method without the network) and the learned start in FP32."""
 import sys
 
