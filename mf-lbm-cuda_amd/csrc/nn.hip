@@ -39,7 +39,8 @@ __device__ __forceinline__ int shifted(const Grid3& g, int m, int tap) {
     return (zz * g.H + yy) * g.W + xx;
 }
 
-// Forward / data-gradient convolution.  One wave: 16 voxels x NT*16 output channels.
+// Forward / data-gradient convolution.  One wave: 16 voxels x NT*16 output channels, the
+// 16-channel tiles blockIdx.y * NT .. + NT - 1 (small grids split the channels over blocks).
 // Epilogue: + bias (if given), then either ReLU, or a multiply by the ReLU mask of `gate`
 // (the data gradient of a layer whose input came through a ReLU), stored as bf16 and/or fp32.
 template <int NT>
@@ -54,6 +55,7 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, con
     const int r = lane & 15, q = lane >> 4;
     const int m = m0 + r;
     const int K = 27 * cin;
+    const int tb = blockIdx.y * NT;  // first channel tile of this block
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -69,13 +71,13 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, con
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wb + (int64_t)(t * 16 + r) * kp + k);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wb + (int64_t)((tb + t) * 16 + r) * kp + k);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int n = t * 16 + r;
+        const int n = (tb + t) * 16 + r;
         if (n >= cout) continue;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -91,10 +93,12 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, con
 }
 
 // Weight gradient partials.  Block (kt, nt, chunk): one wave per 16 x 16 tile of
-// (output channel n, k) over `chunk` voxels; writes part[chunk][n][k] (fp32).
+// (output channel n, k) over `chunk` voxels; writes part[chunk][n][k] (fp32).  The waves of
+// the first k tile also sum the dZ values they load: the bias gradient partials
+// bpart[chunk][n] (n < cout; null: not wanted).
 __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X, const __bf16* __restrict__ dZ,
                                                     Grid3 g, int cin, int kp, int coutp, int cout, int chunk,
-                                                    float* __restrict__ part) {
+                                                    float* __restrict__ part, float* __restrict__ bpart) {
     const int M = g.D * g.H * g.W;
     const int lane = threadIdx.x;
     const int k0 = blockIdx.x * 16, n0 = blockIdx.y * 16, c = blockIdx.z;
@@ -104,19 +108,40 @@ __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X
     int tap = 0, ci = 0;
     if (kk < K) { tap = kk / cin; ci = kk - tap * cin; }
     const int n = n0 + r;   // this lane's A row
+    // the lane's tap as a coordinate shift and as a voxel offset
+    const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
+    const int off = (oz * g.H + oy) * g.W + ox;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;  // this lane's dZ values (row n), in voxel order
     const int mb = c * chunk, me = min(M, mb + chunk);
     for (int ms = mb; ms < me; ms += 32) {
+        // a lane's 8 voxels are consecutive: divide once, then step the coordinates along x
+        const int m0 = ms + 8 * q;
+        int x = m0 % g.W, y = (m0 / g.W) % g.H, z = m0 / (g.W * g.H);
         bf16x8 a, b;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int m = ms + 8 * q + j;
+            const int m = m0 + j;
             a[j] = (m < me && n < cout) ? dZ[(int64_t)m * cout + n] : (__bf16)0.f;
-            int s = -1;
-            if (m < me && kk < K) s = shifted(g, m, tap);
-            b[j] = s >= 0 ? X[(int64_t)s * cin + ci] : (__bf16)0.f;
+            const int xx = x + ox, yy = y + oy, zz = z + oz;
+            const bool in = m < me && kk < K && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D;
+            b[j] = in ? X[(int64_t)(m + off) * cin + ci] : (__bf16)0.f;
+            if (++x == g.W) {
+                x = 0;
+                if (++y == g.H) { y = 0; ++z; }
+            }
+        }
+        if (bpart && k0 == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bsum += bf2f(a[j]);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (bpart && k0 == 0) {
+        // lanes r, r + 16, r + 32, r + 48 hold row n: fixed pairing order
+        const float s1 = bsum + __shfl_xor(bsum, 16);
+        const float s2 = s1 + __shfl_xor(s1, 32);
+        if (q == 0 && n < cout) bpart[(int64_t)c * cout + n] = s2;
     }
     // D[row = n0 + 4q + j][col = k0 + r]
 #pragma unroll
@@ -131,22 +156,6 @@ __global__ void k_sum_chunks(const float* __restrict__ part, int nchunk, int64_t
     float s = 0.f;
     for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
     dw[i] = accumulate ? dw[i] + s : s;
-}
-
-// db[n] (+)= sum over voxels of dZ[m][n] (one block per channel, fixed-order tree).
-__global__ __launch_bounds__(256) void k_bias_grad(const __bf16* __restrict__ dZ, int M, int cout, int accumulate,
-                                                   float* __restrict__ db) {
-    const int n = blockIdx.x;
-    float s = 0.f;
-    for (int m = threadIdx.x; m < M; m += 256) s += bf2f(dZ[(int64_t)m * cout + n]);
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[n] = accumulate ? db[n] + red[0] : red[0];
 }
 
 // W'[ci][tap' * cout + co] = W[co][(26 - tap') * cin + ci]  (bf16, padded: ci to 16, k' to 32)
@@ -314,14 +323,21 @@ int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate
     if (cin % 8 || kp % 32 || kp < 27 * cin || cout <= 0 || cout > 192 || D <= 0 || H <= 0 || W <= 0) return -1;
     const Grid3 g{D, H, W};
     const int64_t M = (int64_t)D * H * W;
-    const unsigned grid = nblk(M, 64);
+    const unsigned vblocks = nblk(M, 64);
     hipStream_t s = (hipStream_t)stream;
-    const int nt = (cout + 15) / 16;
+    // channel tiles the weight rows cover: ceil(cout / 16), or 12 (192 rows) above 128 channels
+    const int ntw = cout > 128 ? 12 : (cout + 15) / 16;
     const __bf16* X = (const __bf16*)x;
     const __bf16* Wb = (const __bf16*)w;
     const __bf16* G = (const __bf16*)gate;
     __bf16* Y = (__bf16*)yb;
-    switch (nt) {
+    if (vblocks < 256) {
+        // fewer voxel blocks than CUs: one channel tile per block, the tiles over grid.y
+        k_conv3<1><<<dim3(vblocks, ntw), 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+        return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
+    const unsigned grid = vblocks;
+    switch (ntw) {
         case 1: k_conv3<1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
         case 2: k_conv3<2><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
         case 3: k_conv3<3><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
@@ -338,7 +354,7 @@ int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate
 int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_t cout, int32_t chunk) {
     const int64_t M = (int64_t)D * H * W;
     const int64_t nchunk = (M + chunk - 1) / chunk;
-    return nchunk * (int64_t)((cout + 15) / 16 * 16) * kp;
+    return nchunk * ((int64_t)((cout + 15) / 16 * 16) * kp + cout);  // weight partials, then bias partials
 }
 
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
@@ -350,10 +366,12 @@ int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_
     const int coutp = (cout + 15) / 16 * 16;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid(kp / 16, coutp / 16, nchunk);
-    k_conv3_wgrad<<<grid, 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g, cin, kp, coutp, cout, chunk, work);
     const int64_t n = (int64_t)coutp * kp;
+    float* bpart = db ? work + nchunk * n : nullptr;
+    k_conv3_wgrad<<<grid, 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g, cin, kp, coutp, cout, chunk, work,
+                                      bpart);
     k_sum_chunks<<<nblk(n), 256, 0, s>>>(work, nchunk, n, accumulate, dw);
-    if (db) k_bias_grad<<<cout, 256, 0, s>>>((const __bf16*)dz, (int)M, cout, accumulate, db);
+    if (db) k_sum_chunks<<<nblk(cout), 256, 0, s>>>(bpart, nchunk, cout, accumulate, db);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

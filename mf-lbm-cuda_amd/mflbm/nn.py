@@ -50,6 +50,10 @@ SIGNATURES = {
 }
 
 
+# voxels per weight-gradient chunk: each chunk's fp32 partials are summed in chunk order
+WGRAD_CHUNK = 2048
+
+
 def lib():
     global _bound
     L = gpu.load()
@@ -199,7 +203,7 @@ class Conv:
         (times [gate > 0] when gate is given)."""
         D, H, W = shape
         L = lib()
-        _chk(L.mfnn_conv3_wgrad(ptr(x), ptr(dz), D, H, W, self.cin, self.kp, self.cout, 2048, ptr(work),
+        _chk(L.mfnn_conv3_wgrad(ptr(x), ptr(dz), D, H, W, self.cin, self.kp, self.cout, WGRAD_CHUNK, ptr(work),
                                 ptr(self.dw), ptr(self.db), int(accumulate), stream), "wgrad")
         if dx is not None:
             _chk(L.mfnn_flip_weights(ptr(self.wb), self.cin, self.kp, self.cout, self.kp2, self.wt.shape[0],
@@ -252,7 +256,7 @@ class UNet:
         need = 0
         for k, c in self.conv.items():
             s = 1 if k in ("e1", "e2", "d11", "head") else (2 if k in ("e3", "e4", "d10") else (4 if k in ("e5", "e6", "d9") else 8))
-            need = max(need, L.mfnn_wgrad_workspace(D // s, H // s, W // s, c.kp, c.cout, 2048))
+            need = max(need, L.mfnn_wgrad_workspace(D // s, H // s, W // s, c.kp, c.cout, WGRAD_CHUNK))
         if self._work is None or self._work.numel() < need:
             self._work = DevArray((need,), "f32")
         return self._work

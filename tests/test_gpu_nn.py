@@ -60,7 +60,7 @@ def test_conv_forward_dgrad_wgrad(nn, cin, cout, shape):
 
     dz = bf16_round(rng.normal(size=shape + (cout,)))
     dzd = nn.DevArray.from_numpy(dz, "bf16")
-    work = nn.DevArray((nn.lib().mfnn_wgrad_workspace(*shape, conv.kp, cout, 2048),), "f32")
+    work = nn.DevArray((nn.lib().mfnn_wgrad_workspace(*shape, conv.kp, cout, nn.WGRAD_CHUNK),), "f32")
     dx = nn.DevArray(shape + (cin,), "bf16")
     conv.backward(xd, dzd, shape, work, accumulate=False, dx=dx)
     dw_ref, db_ref = N.conv3_wgrad(x, dz)
@@ -151,3 +151,17 @@ def test_training_reduces_loss(nn):
     say(f"  loss {losses[0]:.4e} -> {losses[-1]:.4e}")
     assert losses[-1] < 0.1 * losses[0]
     assert np.all(net.conv["head"].w.numpy()[4:8] == 0)
+
+
+def test_nn_bench_runs(nn):
+    """The convolution and training-step benchmark runs end to end at 16^3 and reports every
+    layer with positive, finite rates."""
+    say("nn_bench 16^3")
+    from mflbm import nn_bench
+    out = nn_bench.run(n=16, iters=2, train_steps=2, log=lambda s: None)
+    assert [r["layer"] for r in out["layers"]] == [name for name, _, _ in nn_bench.LAYERS]
+    for r in out["layers"]:
+        assert np.isfinite(r["fwd_tflops"]) and r["fwd_tflops"] > 0
+        assert np.isfinite(r["wgrad_tflops"]) and r["wgrad_tflops"] > 0
+        assert ("dgrad_ms" in r) == (r["layer"] != "e1")
+    assert out["train_step_ms"] > 0 and np.isfinite(out["train_tflops"])
