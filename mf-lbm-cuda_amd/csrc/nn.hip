@@ -92,40 +92,54 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, con
     }
 }
 
-// Weight gradient partials.  Block (kt, nt, chunk): one wave per 16 x 16 tile of
-// (output channel n, k) over `chunk` voxels; writes part[chunk][n][k] (fp32).  The waves of
-// the first k tile also sum the dZ values they load: the bias gradient partials
-// bpart[chunk][n] (n < cout; null: not wanted).
+// Weight gradient partials.  Block (kt, nt, chunk): one wave per 16 x 16·KT block of (output
+// channel n, k) over `chunk` voxels, as KT 16 x 16 MFMA tiles that share the dZ operand;
+// writes part[chunk][n][k] (fp32).  The waves of the first k block also sum the dZ values
+// they load: the bias gradient partials bpart[chunk][n] (n < cout; null: not wanted).
+template <int KT>
 __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X, const __bf16* __restrict__ dZ,
                                                     Grid3 g, int cin, int kp, int coutp, int cout, int chunk,
                                                     float* __restrict__ part, float* __restrict__ bpart) {
     const int M = g.D * g.H * g.W;
     const int lane = threadIdx.x;
-    const int k0 = blockIdx.x * 16, n0 = blockIdx.y * 16, c = blockIdx.z;
+    const int k0 = blockIdx.x * 16 * KT, n0 = blockIdx.y * 16, c = blockIdx.z;
     const int r = lane & 15, q = lane >> 4;
     const int K = 27 * cin;
-    const int kk = k0 + r;  // this lane's B column
-    int tap = 0, ci = 0;
-    if (kk < K) { tap = kk / cin; ci = kk - tap * cin; }
-    const int n = n0 + r;   // this lane's A row
-    // the lane's tap as a coordinate shift and as a voxel offset
-    const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
-    const int off = (oz * g.H + oy) * g.W + ox;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int n = n0 + r;  // this lane's A row
+    // this lane's B column in each tile, its tap as a coordinate shift and a voxel offset
+    int ci[KT], ox[KT], oy[KT], oz[KT], off[KT];
+    bool kin[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        const int kk = k0 + 16 * t + r;
+        kin[t] = kk < K;
+        const int tap = kin[t] ? kk / cin : 0;
+        ci[t] = kin[t] ? kk - tap * cin : 0;
+        ox[t] = tap % 3 - 1;
+        oy[t] = (tap / 3) % 3 - 1;
+        oz[t] = tap / 9 - 1;
+        off[t] = (oz[t] * g.H + oy[t]) * g.W + ox[t];
+    }
+    f32x4 acc[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;  // this lane's dZ values (row n), in voxel order
     const int mb = c * chunk, me = min(M, mb + chunk);
     for (int ms = mb; ms < me; ms += 32) {
         // a lane's 8 voxels are consecutive: divide once, then step the coordinates along x
         const int m0 = ms + 8 * q;
         int x = m0 % g.W, y = (m0 / g.W) % g.H, z = m0 / (g.W * g.H);
-        bf16x8 a, b;
+        bf16x8 a, b[KT];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int m = m0 + j;
             a[j] = (m < me && n < cout) ? dZ[(int64_t)m * cout + n] : (__bf16)0.f;
-            const int xx = x + ox, yy = y + oy, zz = z + oz;
-            const bool in = m < me && kk < K && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D;
-            b[j] = in ? X[(int64_t)(m + off) * cin + ci] : (__bf16)0.f;
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int xx = x + ox[t], yy = y + oy[t], zz = z + oz[t];
+                const bool in = m < me && kin[t] && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D;
+                b[t][j] = in ? X[(int64_t)(m + off[t]) * cin + ci[t]] : (__bf16)0.f;
+            }
             if (++x == g.W) {
                 x = 0;
                 if (++y == g.H) { y = 0; ++z; }
@@ -135,7 +149,8 @@ __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X
 #pragma unroll
             for (int j = 0; j < 8; ++j) bsum += bf2f(a[j]);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[t], 0, 0, 0);
     }
     if (bpart && k0 == 0) {
         // lanes r, r + 16, r + 32, r + 48 hold row n: fixed pairing order
@@ -143,19 +158,32 @@ __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X
         const float s2 = s1 + __shfl_xor(s1, 32);
         if (q == 0 && n < cout) bpart[(int64_t)c * cout + n] = s2;
     }
-    // D[row = n0 + 4q + j][col = k0 + r]
+    // tile t: D[row = n0 + 4q + j][col = k0 + 16t + r]
 #pragma unroll
-    for (int j = 0; j < 4; ++j) part[((int64_t)c * coutp + n0 + 4 * q + j) * kp + k0 + r] = acc[j];
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part[((int64_t)c * coutp + n0 + 4 * q + j) * kp + k0 + 16 * t + r] = acc[t][j];
 }
 
-// dW[n][k] (+)= sum over chunks of part[chunk][n][k], in chunk order.
-__global__ void k_sum_chunks(const float* __restrict__ part, int nchunk, int64_t n, int accumulate,
-                             float* __restrict__ dw) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// dW[i] (+)= sum over chunks of part[chunk][i], i < n.  A block covers 32 entries with 8
+// groups of 32 threads: group g adds chunks g, g + 8, g + 16, ... in order, then the 8 group
+// sums are added in group order (a fixed order for every launch).
+__global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part, int nchunk, int64_t n,
+                                                    int accumulate, float* __restrict__ dw) {
+    __shared__ float red[8][32];
+    const int x = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const int64_t i = (int64_t)blockIdx.x * 32 + x;
     float s = 0.f;
-    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
-    dw[i] = accumulate ? dw[i] + s : s;
+    if (i < n)
+        for (int c = grp; c < nchunk; c += 8) s += part[(int64_t)c * n + i];
+    red[grp][x] = s;
+    __syncthreads();
+    if (grp == 0 && i < n) {
+        float t = red[0][x];
+#pragma unroll
+        for (int g = 1; g < 8; ++g) t += red[g][x];
+        dw[i] = accumulate ? dw[i] + t : t;
+    }
 }
 
 // W'[ci][tap' * cout + co] = W[co][(26 - tap') * cin + ci]  (bf16, padded: ci to 16, k' to 32)
@@ -365,13 +393,18 @@ int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_
     const int nchunk = (int)((M + chunk - 1) / chunk);
     const int coutp = (cout + 15) / 16 * 16;
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid(kp / 16, coutp / 16, nchunk);
     const int64_t n = (int64_t)coutp * kp;
     float* bpart = db ? work + nchunk * n : nullptr;
-    k_conv3_wgrad<<<grid, 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g, cin, kp, coutp, cout, chunk, work,
-                                      bpart);
-    k_sum_chunks<<<nblk(n), 256, 0, s>>>(work, nchunk, n, accumulate, dw);
-    if (db) k_sum_chunks<<<nblk(cout), 256, 0, s>>>(bpart, nchunk, cout, accumulate, db);
+    // two k tiles per wave halve the dZ loads, one keeps twice the waves: the second wins
+    // when the single-tile grid has fewer than 3072 waves (three per SIMD)
+    if ((int64_t)(kp / 16) * (coutp / 16) * nchunk >= 3072)  // kp is a multiple of 32
+        k_conv3_wgrad<2><<<dim3(kp / 32, coutp / 16, nchunk), 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g,
+                                                                          cin, kp, coutp, cout, chunk, work, bpart);
+    else
+        k_conv3_wgrad<1><<<dim3(kp / 16, coutp / 16, nchunk), 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g,
+                                                                          cin, kp, coutp, cout, chunk, work, bpart);
+    k_sum_chunks<<<nblk(n, 32), 256, 0, s>>>(work, nchunk, n, accumulate, dw);
+    if (db) k_sum_chunks<<<nblk(cout, 32), 256, 0, s>>>(bpart, nchunk, cout, accumulate, db);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
