@@ -32,13 +32,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
+#include <functional>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
 #include <vector>
 
 #include "mflbm.h"
@@ -356,6 +359,7 @@ public:
         ntime_monitor_profile_ = c.monitor_profile_ratio * c.ntime_monitor;
     }
     ~Simulation() {
+        if (writer_.joinable()) writer_.join();
         if (h_) mflbm_destroy(h_);
     }
 
@@ -397,6 +401,30 @@ private:
         if (!f.good()) throw Fatal("Could not open results/out1.output/" + name);
         return f;
     }
+
+    // File output runs on one background thread: the device state is downloaded at the output
+    // step, then formatted and written while the next steps run.  One writer at a time, in
+    // output order; a writer's error surfaces at the next output or at the end of the run.
+    void drain() {
+        if (writer_.joinable()) writer_.join();
+        if (writer_err_) {
+            const std::exception_ptr e = writer_err_;
+            writer_err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    void background(std::function<void()> job) {
+        drain();
+        writer_ = std::thread([this, job = std::move(job)]() {
+            try {
+                job();
+            } catch (...) {
+                writer_err_ = std::current_exception();
+            }
+        });
+    }
+    std::thread writer_;
+    std::exception_ptr writer_err_;
 
     Control c_;  // a copy: the velocity inlet rescales the timers by the injected volume
     const int prec_;
@@ -583,39 +611,44 @@ void Simulation::load_checkpoint(double* force_z, double* rho_in, std::vector<do
 }
 
 void Simulation::save_checkpoint(int where) {
+    drain();  // one set of output buffers at a time
     const std::string path = where == 0 ? "results/out2.checkpoint/id0000" : "results/out2.checkpoint/2rd_backup/id0000";
-    FILE* fp = std::fopen(path.c_str(), "wb+");
-    if (!fp) throw Fatal("Could not create save_checkpoint file!");
     const int32_t nt = ntime_ + 1;
-    bool ok = std::fwrite(&nt, 4, 1, fp) == 1;
-    auto put = [&](const void* data, size_t n) { ok = ok && std::fwrite(data, es_, n, fp) == n; };
-    std::vector<char> pdf((size_t)(38 * s1_) * es_), phi((size_t)s4_ * es_);
+    std::vector<char> pdf((size_t)(38 * s1_) * es_), phi((size_t)s4_ * es_), conv;
     check(mflbm_download(h_, MFLBM_PDF, pdf.data(), pdf.size()), "download pdf");
     check(mflbm_download(h_, MFLBM_PHI, phi.data(), phi.size()), "download phi");
-    if (prec_ == MFLBM_FP64) {
-        put(&p_.force_z, 1);
-        put(&p_.rho_in, 1);
-    } else {
-        const float fz = (float)p_.force_z, ri = (float)p_.rho_in;
-        put(&fz, 1);
-        put(&ri, 1);
-    }
-    put(pdf.data(), (size_t)(38 * s1_));
-    put(phi.data(), (size_t)s4_);
     if (c_.convective_outlet()) {
-        const size_t n = (size_t)(39 * (nx_ + 2) * (ny_ + 2));
-        std::vector<char> conv(n * es_);
+        conv.resize((size_t)(39 * (nx_ + 2) * (ny_ + 2)) * es_);
         check(mflbm_download(h_, MFLBM_CONVEC, conv.data(), conv.size()), "download convective outlet state");
-        put(conv.data(), n);
     }
-    std::fclose(fp);
-    if (!ok) throw Fatal("Could not write the checkpoint file " + path);
-    if (where == 0) {
-        std::cout << "Saving checkpoint data completed!" << std::endl;
-        write_status("continue_simulation");
-    } else {
-        std::cout << "Saving secondary checkpoint data completed!" << std::endl;
-    }
+    const double fz = p_.force_z, ri = p_.rho_in;
+    const size_t es = es_;
+    const bool fp64 = prec_ == MFLBM_FP64;
+    background([=, pdf = std::move(pdf), phi = std::move(phi), conv = std::move(conv)]() {
+        FILE* fp = std::fopen(path.c_str(), "wb+");
+        if (!fp) throw Fatal("Could not create save_checkpoint file!");
+        bool ok = std::fwrite(&nt, 4, 1, fp) == 1;
+        auto put = [&](const void* data, size_t n) { ok = ok && std::fwrite(data, es, n, fp) == n; };
+        if (fp64) {
+            put(&fz, 1);
+            put(&ri, 1);
+        } else {
+            const float f = (float)fz, r = (float)ri;
+            put(&f, 1);
+            put(&r, 1);
+        }
+        put(pdf.data(), pdf.size() / es);
+        put(phi.data(), phi.size() / es);
+        if (!conv.empty()) put(conv.data(), conv.size() / es);
+        std::fclose(fp);
+        if (!ok) throw Fatal("Could not write the checkpoint file " + path);
+        if (where == 0) {
+            std::cout << "Saving checkpoint data completed!" << std::endl;
+            write_status("continue_simulation");
+        } else {
+            std::cout << "Saving secondary checkpoint data completed!" << std::endl;
+        }
+    });
 }
 
 // Monitor of an unsteady (displacement) run: src/Monitor.cpp:17-277, sums from the device.
@@ -787,47 +820,55 @@ void Simulation::monitor_breakthrough() {
 // Legacy binary VTK, big endian (src/IO_multiphase.cpp:500-716).  Type 1: phi, density and
 // velocity; type 2: phi (float) with phi = 0 at solid nodes.
 void Simulation::vtk(int type) {
+    drain();  // one set of output buffers at a time
     std::ostringstream name;
     name << std::setfill('0') << std::setw(10) << ntime_ << ".vtk";
     const std::string path = type == 1 ? "results/out3.field_data/full_flow_field/full_flow_field_" + name.str()
                                        : "results/out3.field_data/phase_distribution/small_" + name.str();
-    std::ofstream out(path, std::ios::out | std::ios::binary);
-    if (!out.good()) throw Fatal("Could not open vtk outputl file!");
-    const int64_t n = nx_ * ny_ * nz_;
-    out << "# vtk DataFile Version 3.0" << std::endl << "vtk output" << std::endl << "BINARY" << std::endl
-        << "DATASET STRUCTURED_POINTS" << std::endl
-        << "DIMENSIONS " << nx_ << " " << ny_ << " " << nz_ << std::endl
-        << "ORIGIN 1 1 1" << std::endl << "SPACING 1 1 1" << std::endl << "POINT_DATA " << n << std::endl;
-    const std::vector<double> phi = download_real(MFLBM_PHI, s4_);
-    if (type == 2) {
-        out << "SCALARS phi float" << std::endl << "LOOKUP_TABLE default" << std::endl;
-        for (int64_t k = 1; k <= nz_; ++k)
-            for (int64_t j = 1; j <= ny_; ++j)
-                for (int64_t i = 1; i <= nx_; ++i) {
-                    const int w = walls_[i_s2(i, j, k)];
-                    put_be(out, (float)(0.0 * w + phi[i_s4(i, j, k)] * (1 - w)));
-                }
-        return;
+    std::vector<double> phi = download_real(MFLBM_PHI, s4_), rho, u, v, w;
+    if (type == 1) {
+        rho.resize((size_t)s1_); u.resize((size_t)s1_); v.resize((size_t)s1_); w.resize((size_t)s1_);
+        check(mflbm_macro(h_, rho.data(), u.data(), v.data(), w.data()), "macro");
     }
-    std::vector<double> rho((size_t)s1_), u((size_t)s1_), v((size_t)s1_), w((size_t)s1_);
-    check(mflbm_macro(h_, rho.data(), u.data(), v.data(), w.data()), "macro");
-    const bool dbl = c_.field_precision != 0;
-    const char* fmt = dbl ? "double" : "float";
-    auto block = [&](const char* label, auto value) {
-        out << "SCALARS " << label << " " << fmt << std::endl << "LOOKUP_TABLE default" << std::endl;
-        for (int64_t k = 1; k <= nz_; ++k)
-            for (int64_t j = 1; j <= ny_; ++j)
-                for (int64_t i = 1; i <= nx_; ++i) {
-                    const double x = value(i, j, k);
-                    if (dbl) put_be(out, x);
-                    else put_be(out, (float)x);
-                }
-    };
-    block("phi", [&](int64_t i, int64_t j, int64_t k) { return phi[i_s4(i, j, k)]; });
-    block("density", [&](int64_t i, int64_t j, int64_t k) { return rho[i_s1(i, j, k)]; });
-    block("velocity_X", [&](int64_t i, int64_t j, int64_t k) { return u[i_s1(i, j, k)]; });
-    block("velocity_Y", [&](int64_t i, int64_t j, int64_t k) { return v[i_s1(i, j, k)]; });
-    block("velocity_Z", [&](int64_t i, int64_t j, int64_t k) { return w[i_s1(i, j, k)]; });
+    // walls_ and the sizes do not change after set-up, so the writer reads them in place
+    background([this, type, path, phi = std::move(phi), rho = std::move(rho), u = std::move(u), v = std::move(v),
+                w = std::move(w)]() {
+        std::ofstream out(path, std::ios::out | std::ios::binary);
+        if (!out.good()) throw Fatal("Could not open vtk outputl file!");
+        const int64_t n = nx_ * ny_ * nz_;
+        out << "# vtk DataFile Version 3.0" << std::endl << "vtk output" << std::endl << "BINARY" << std::endl
+            << "DATASET STRUCTURED_POINTS" << std::endl
+            << "DIMENSIONS " << nx_ << " " << ny_ << " " << nz_ << std::endl
+            << "ORIGIN 1 1 1" << std::endl << "SPACING 1 1 1" << std::endl << "POINT_DATA " << n << std::endl;
+        if (type == 2) {
+            out << "SCALARS phi float" << std::endl << "LOOKUP_TABLE default" << std::endl;
+            for (int64_t k = 1; k <= nz_; ++k)
+                for (int64_t j = 1; j <= ny_; ++j)
+                    for (int64_t i = 1; i <= nx_; ++i) {
+                        const int wl = walls_[i_s2(i, j, k)];
+                        put_be(out, (float)(0.0 * wl + phi[i_s4(i, j, k)] * (1 - wl)));
+                    }
+            return;
+        }
+        const bool dbl = c_.field_precision != 0;
+        const char* fmt = dbl ? "double" : "float";
+        auto block = [&](const char* label, auto value) {
+            out << "SCALARS " << label << " " << fmt << std::endl << "LOOKUP_TABLE default" << std::endl;
+            for (int64_t k = 1; k <= nz_; ++k)
+                for (int64_t j = 1; j <= ny_; ++j)
+                    for (int64_t i = 1; i <= nx_; ++i) {
+                        const double x = value(i, j, k);
+                        if (dbl) put_be(out, x);
+                        else put_be(out, (float)x);
+                    }
+        };
+        block("phi", [&](int64_t i, int64_t j, int64_t k) { return phi[i_s4(i, j, k)]; });
+        block("density", [&](int64_t i, int64_t j, int64_t k) { return rho[i_s1(i, j, k)]; });
+        block("velocity_X", [&](int64_t i, int64_t j, int64_t k) { return u[i_s1(i, j, k)]; });
+        block("velocity_Y", [&](int64_t i, int64_t j, int64_t k) { return v[i_s1(i, j, k)]; });
+        block("velocity_Z", [&](int64_t i, int64_t j, int64_t k) { return w[i_s1(i, j, k)]; });
+        if (!out.good()) throw Fatal("Could not write " + path);
+    });
 }
 
 void Simulation::vtk_walls() const {
@@ -971,20 +1012,24 @@ int Simulation::run(const std::string& job_status) {
         save_checkpoint(0);
         vtk(2);
         vtk(1);
+        drain();
         std::cout << "Simulation ended after\t" << ntime_ << " iterations which reached the maximum time step!" << std::endl;
         write_status("simulation_reached_max_step");
     } else if (end_ == 1) {
         save_checkpoint(0);
         vtk(2);
         vtk(1);
+        drain();
         std::cout << "Simulation ended successfully after\t" << ntime_ << " iterations!" << std::endl;
         write_status("simulation_done");
     } else if (end_ == 2) {
         save_checkpoint(0);
+        drain();
         std::cout << "Simulation data saved but not finished after\t" << ntime_ << " iterations!" << std::endl;
         write_status("continue_simulation");
     } else {
         vtk(1);
+        drain();
         std::cout << "Simulation failed after\t" << ntime_ << " iterations!" << std::endl;
         write_status("simulation_failed");
     }
