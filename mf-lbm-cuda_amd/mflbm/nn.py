@@ -50,8 +50,14 @@ SIGNATURES = {
 }
 
 
-# voxels per weight-gradient chunk: each chunk's fp32 partials are summed in chunk order
+# voxels per weight-gradient chunk: each chunk's fp32 partials are summed in chunk order.
+# Grids below 65536 voxels use WGRAD_CHUNK_SMALL, so that they still spread over many waves.
 WGRAD_CHUNK = 2048
+WGRAD_CHUNK_SMALL = 512
+
+
+def wgrad_chunk(D, H, W):
+    return WGRAD_CHUNK if D * H * W >= 65536 else WGRAD_CHUNK_SMALL
 
 
 def lib():
@@ -203,8 +209,8 @@ class Conv:
         (times [gate > 0] when gate is given)."""
         D, H, W = shape
         L = lib()
-        _chk(L.mfnn_conv3_wgrad(ptr(x), ptr(dz), D, H, W, self.cin, self.kp, self.cout, WGRAD_CHUNK, ptr(work),
-                                ptr(self.dw), ptr(self.db), int(accumulate), stream), "wgrad")
+        _chk(L.mfnn_conv3_wgrad(ptr(x), ptr(dz), D, H, W, self.cin, self.kp, self.cout, wgrad_chunk(D, H, W),
+                                ptr(work), ptr(self.dw), ptr(self.db), int(accumulate), stream), "wgrad")
         if dx is not None:
             _chk(L.mfnn_flip_weights(ptr(self.wb), self.cin, self.kp, self.cout, self.kp2, self.wt.shape[0],
                                      ptr(self.wt), stream), "flip")
@@ -256,7 +262,8 @@ class UNet:
         need = 0
         for k, c in self.conv.items():
             s = 1 if k in ("e1", "e2", "d11", "head") else (2 if k in ("e3", "e4", "d10") else (4 if k in ("e5", "e6", "d9") else 8))
-            need = max(need, L.mfnn_wgrad_workspace(D // s, H // s, W // s, c.kp, c.cout, WGRAD_CHUNK))
+            d, h, w = D // s, H // s, W // s
+            need = max(need, L.mfnn_wgrad_workspace(d, h, w, c.kp, c.cout, wgrad_chunk(d, h, w)))
         if self._work is None or self._work.numel() < need:
             self._work = DevArray((need,), "f32")
         return self._work

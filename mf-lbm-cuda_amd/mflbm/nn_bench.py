@@ -88,7 +88,7 @@ def run(n=64, iters=20, train_steps=10, widths=(16, 32, 64, 128), log=print):
     conv_ms = sum(r["fwd_ms"] + r["wgrad_ms"] + r.get("dgrad_ms", 0.0) for r in rows)
     best = max(rows, key=lambda r: r["gflop"])
     out = {"metric": "unet_train_step", "n": n, "widths": list(widths), "params": net.n_params(),
-           "wgrad_chunk": nn.WGRAD_CHUNK,
+           "wgrad_chunk": [nn.WGRAD_CHUNK, nn.WGRAD_CHUNK_SMALL],
            "train_step_ms": ms, "train_tflops": total_flops / ms / 1e9,
            "train_frac": total_flops / ms / 1e9 / PEAK_BF16_TFLOPS, "conv_ms_sum": conv_ms,
            "largest_layer": {"layer": best["layer"], "fwd_tflops": best["fwd_tflops"],
@@ -103,9 +103,12 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=10)
-    ap.add_argument("--chunk", type=int, default=nn.WGRAD_CHUNK, help="voxels per weight-gradient chunk")
+    ap.add_argument("--chunk", type=int, default=nn.WGRAD_CHUNK,
+                    help="voxels per weight-gradient chunk on grids of 65536 voxels and more")
+    ap.add_argument("--small-chunk", type=int, default=nn.WGRAD_CHUNK_SMALL,
+                    help="voxels per weight-gradient chunk on smaller grids")
     args = ap.parse_args(argv)
-    nn.WGRAD_CHUNK = args.chunk
+    nn.WGRAD_CHUNK, nn.WGRAD_CHUNK_SMALL = args.chunk, args.small_chunk
     out = run(args.n, args.iters, args.train_steps, log=lambda s: print(s, file=sys.stderr, flush=True))
     print(json.dumps(out))
 

@@ -60,7 +60,7 @@ def test_conv_forward_dgrad_wgrad(nn, cin, cout, shape):
 
     dz = bf16_round(rng.normal(size=shape + (cout,)))
     dzd = nn.DevArray.from_numpy(dz, "bf16")
-    work = nn.DevArray((nn.lib().mfnn_wgrad_workspace(*shape, conv.kp, cout, nn.WGRAD_CHUNK),), "f32")
+    work = nn.DevArray((nn.lib().mfnn_wgrad_workspace(*shape, conv.kp, cout, nn.wgrad_chunk(*shape)),), "f32")
     dx = nn.DevArray(shape + (cin,), "bf16")
     conv.backward(xd, dzd, shape, work, accumulate=False, dx=dx)
     dw_ref, db_ref = N.conv3_wgrad(x, dz)
