@@ -236,9 +236,13 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 // odd, 395-726 even).  Odd: pull q from (x - e_q).slot q, push q to (x + e_q).slot opp(q);
 // even: read q from x.slot opp(q), write q to x.slot q.  Each cell is read and written by
 // exactly one node per sweep, so the update is in place.
+#ifndef MFLBM_COLLIDE_THREADS
+#define MFLBM_COLLIDE_THREADS 256  // threads per collide workgroup (tuning: 128, 256, 512)
+#endif
 template <typename T, bool ODD>
-__global__ __launch_bounds__(256) void k_collide(Sparse<T> S, T* __restrict__ phi, Dims d, StepConst<T> P) {
-    const int t = xcd_block() * 256 + threadIdx.x;
+__global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, T* __restrict__ phi, Dims d,
+                                                                   StepConst<T> P) {
+    const int t = xcd_block() * MFLBM_COLLIDE_THREADS + threadIdx.x;
     if (t >= S.nf) return;
     T g1[19], g2[19], o1[19], o2[19];
     int nb[19];

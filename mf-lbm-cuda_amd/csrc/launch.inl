@@ -19,8 +19,9 @@ __global__ __launch_bounds__(256) void k_to_real(const double* __restrict__ src,
 template <typename T>
 void Launch<T>::collide(const Sparse<T>& S, T* phi, const Dims& d, const StepConst<T>& P, bool odd, hipStream_t s) {
     if (S.nf <= 0) return;
-    if (odd) dev::k_collide<T, true><<<blocks(S.nf), 256, 0, s>>>(S, phi, d, P);
-    else dev::k_collide<T, false><<<blocks(S.nf), 256, 0, s>>>(S, phi, d, P);
+    constexpr int b = MFLBM_COLLIDE_THREADS;
+    if (odd) dev::k_collide<T, true><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+    else dev::k_collide<T, false><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
 }
 
 template <typename T>
