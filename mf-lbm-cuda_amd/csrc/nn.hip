@@ -96,54 +96,67 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, con
 // channel n, k) over `chunk` voxels, as KT 16 x 16 MFMA tiles that share the dZ operand;
 // writes part[chunk][n][k] (fp32).  The waves of the first k block also sum the dZ values
 // they load: the bias gradient partials bpart[chunk][n] (n < cout; null: not wanted).
+// Per step of 32 voxels the operands are staged through LDS: lane (v, h) loads 8 channels
+// (16 bytes) of voxel v -- dZ channels n0 + 8h .., and of each tile the 8 columns
+// k0 + 16t + 8h .. (8 consecutive channels of one tap, since cin is a multiple of 8) at the
+// tap's shifted voxel -- and lane (r, q) then reads column r of voxels 8q .. 8q + 7, the
+// MFMA operand layout.
 template <int KT>
 __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X, const __bf16* __restrict__ dZ,
                                                     Grid3 g, int cin, int kp, int coutp, int cout, int chunk,
                                                     float* __restrict__ part, float* __restrict__ bpart) {
+    __shared__ __attribute__((aligned(16))) __bf16 sA[32][16];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[KT][32][16];
     const int M = g.D * g.H * g.W;
     const int lane = threadIdx.x;
     const int k0 = blockIdx.x * 16 * KT, n0 = blockIdx.y * 16, c = blockIdx.z;
-    const int r = lane & 15, q = lane >> 4;
+    const int r = lane & 15, q = lane >> 4;  // MFMA role: column r, voxels 8q .. 8q + 7
+    const int v = lane >> 1, h = lane & 1;   // loader role: voxel v, 8-channel half h
     const int K = 27 * cin;
-    const int n = n0 + r;  // this lane's A row
-    // this lane's B column in each tile, its tap as a coordinate shift and a voxel offset
-    int ci[KT], ox[KT], oy[KT], oz[KT], off[KT];
-    bool kin[KT];
+    const int n = n0 + r;
+    const bool ain = n0 + 8 * h < cout;  // cout is a multiple of 8: the whole half or none
+    int lci[KT], lox[KT], loy[KT], loz[KT], loff[KT];
+    bool lkin[KT];
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
-        const int kk = k0 + 16 * t + r;
-        kin[t] = kk < K;
-        const int tap = kin[t] ? kk / cin : 0;
-        ci[t] = kin[t] ? kk - tap * cin : 0;
-        ox[t] = tap % 3 - 1;
-        oy[t] = (tap / 3) % 3 - 1;
-        oz[t] = tap / 9 - 1;
-        off[t] = (oz[t] * g.H + oy[t]) * g.W + ox[t];
+        const int kk = k0 + 16 * t + 8 * h;
+        lkin[t] = kk < K;  // K is a multiple of 8: the whole half or none
+        const int tap = lkin[t] ? kk / cin : 0;
+        lci[t] = lkin[t] ? kk - tap * cin : 0;
+        lox[t] = tap % 3 - 1;
+        loy[t] = (tap / 3) % 3 - 1;
+        loz[t] = tap / 9 - 1;
+        loff[t] = (loz[t] * g.H + loy[t]) * g.W + lox[t];
     }
     f32x4 acc[KT];
 #pragma unroll
     for (int t = 0; t < KT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;  // this lane's dZ values (row n), in voxel order
     const int mb = c * chunk, me = min(M, mb + chunk);
+    bf16x8 zero;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) zero[j] = (__bf16)0.f;
     for (int ms = mb; ms < me; ms += 32) {
-        // a lane's 8 voxels are consecutive: divide once, then step the coordinates along x
-        const int m0 = ms + 8 * q;
-        int x = m0 % g.W, y = (m0 / g.W) % g.H, z = m0 / (g.W * g.H);
+        const int m = ms + v;
+        bf16x8 la = zero;
+        if (m < me && ain) la = *reinterpret_cast<const bf16x8*>(dZ + (int64_t)m * cout + n0 + 8 * h);
+        *reinterpret_cast<bf16x8*>(&sA[v][8 * h]) = la;
+        const int x = m % g.W, y = (m / g.W) % g.H, z = m / (g.W * g.H);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const int xx = x + lox[t], yy = y + loy[t], zz = z + loz[t];
+            const bool in = m < me && lkin[t] && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D;
+            bf16x8 lb = zero;
+            if (in) lb = *reinterpret_cast<const bf16x8*>(X + (int64_t)(m + loff[t]) * cin + lci[t]);
+            *reinterpret_cast<bf16x8*>(&sB[t][v][8 * h]) = lb;
+        }
+        __syncthreads();
         bf16x8 a, b[KT];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int m = m0 + j;
-            a[j] = (m < me && n < cout) ? dZ[(int64_t)m * cout + n] : (__bf16)0.f;
+            a[j] = sA[8 * q + j][r];
 #pragma unroll
-            for (int t = 0; t < KT; ++t) {
-                const int xx = x + ox[t], yy = y + oy[t], zz = z + oz[t];
-                const bool in = m < me && kin[t] && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D;
-                b[t][j] = in ? X[(int64_t)(m + off[t]) * cin + ci[t]] : (__bf16)0.f;
-            }
-            if (++x == g.W) {
-                x = 0;
-                if (++y == g.H) { y = 0; ++z; }
-            }
+            for (int t = 0; t < KT; ++t) b[t][j] = sB[t][8 * q + j][r];
         }
         if (bpart && k0 == 0) {
 #pragma unroll
@@ -151,6 +164,7 @@ __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X
         }
 #pragma unroll
         for (int t = 0; t < KT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[t], 0, 0, 0);
+        __syncthreads();
     }
     if (bpart && k0 == 0) {
         // lanes r, r + 16, r + 32, r + 48 hold row n: fixed pairing order
@@ -387,7 +401,8 @@ int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_
 
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
                      int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream) {
-    if (cin % 8 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return -1;
+    if (cin % 8 || cout % 8 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return -1;
+    if ((uintptr_t)x % 16 || (uintptr_t)dz % 16) return -1;  // 16-byte operand loads
     const Grid3 g{D, H, W};
     const int64_t M = (int64_t)D * H * W;
     const int nchunk = (int)((M + chunk - 1) / chunk);
