@@ -179,24 +179,31 @@ __global__ __launch_bounds__(64) void k_conv3_wgrad(const __bf16* __restrict__ X
         for (int j = 0; j < 4; ++j) part[((int64_t)c * coutp + n0 + 4 * q + j) * kp + k0 + 16 * t + r] = acc[t][j];
 }
 
-// dW[i] (+)= sum over chunks of part[chunk][i], i < n.  A block covers 32 entries with 8
-// groups of 32 threads: group g adds chunks g, g + 8, g + 16, ... in order, then the 8 group
-// sums are added in group order (a fixed order for every launch).
+// dW[i] (+)= sum over chunks of part[chunk][i], i < n, and in the blocks after those,
+// db[i] (+)= the same over bpart[chunk][i], i < nb (bpart null: no bias).  A block covers
+// 32 entries with 8 groups of 32 threads: group g adds chunks g, g + 8, g + 16, ... in order,
+// then the 8 group sums are added in group order (a fixed order for every launch).
 __global__ __launch_bounds__(256) void k_sum_chunks(const float* __restrict__ part, int nchunk, int64_t n,
-                                                    int accumulate, float* __restrict__ dw) {
+                                                    const float* __restrict__ bpart, int nb, int accumulate,
+                                                    float* __restrict__ dw, float* __restrict__ db) {
     __shared__ float red[8][32];
     const int x = threadIdx.x & 31, grp = threadIdx.x >> 5;
-    const int64_t i = (int64_t)blockIdx.x * 32 + x;
+    const int64_t wblocks = (n + 31) / 32;
+    const bool bias = (int64_t)blockIdx.x >= wblocks;
+    const float* src = bias ? bpart : part;
+    float* dst = bias ? db : dw;
+    const int64_t len = bias ? nb : n;
+    const int64_t i = ((int64_t)blockIdx.x - (bias ? wblocks : 0)) * 32 + x;
     float s = 0.f;
-    if (i < n)
-        for (int c = grp; c < nchunk; c += 8) s += part[(int64_t)c * n + i];
+    if (i < len)
+        for (int c = grp; c < nchunk; c += 8) s += src[(int64_t)c * len + i];
     red[grp][x] = s;
     __syncthreads();
-    if (grp == 0 && i < n) {
+    if (grp == 0 && i < len) {
         float t = red[0][x];
 #pragma unroll
         for (int g = 1; g < 8; ++g) t += red[g][x];
-        dw[i] = accumulate ? dw[i] + t : t;
+        dst[i] = accumulate ? dst[i] + t : t;
     }
 }
 
@@ -418,8 +425,8 @@ int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_
     else
         k_conv3_wgrad<1><<<dim3(kp / 16, coutp / 16, nchunk), 64, 0, s>>>((const __bf16*)x, (const __bf16*)dz, g,
                                                                           cin, kp, coutp, cout, chunk, work, bpart);
-    k_sum_chunks<<<nblk(n, 32), 256, 0, s>>>(work, nchunk, n, accumulate, dw);
-    if (db) k_sum_chunks<<<nblk(cout, 32), 256, 0, s>>>(bpart, nchunk, cout, accumulate, db);
+    const unsigned sblocks = nblk(n, 32) + (db ? nblk(cout, 32) : 0);
+    k_sum_chunks<<<sblocks, 256, 0, s>>>(work, nchunk, n, bpart, db ? cout : 0, accumulate, dw, db);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
