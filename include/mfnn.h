@@ -40,7 +40,8 @@ int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate
 
 /* dw[n][k] (+)= sum_voxels dz[m][n] * x[m + off(tap)][ci] and db[n] (+)= sum_m dz[m][n]
  * (accumulate != 0 adds to the existing values).  The voxel sum runs in chunks of `chunk`
- * voxels (multiple of 32) whose fp32 partials go to `work` (mfnn_wgrad_workspace floats). */
+ * voxels (multiple of 32) whose fp32 partials go to `work` (mfnn_wgrad_workspace floats).
+ * cin and cout are multiples of 8 and x, dz 16-byte aligned (returns -1 otherwise). */
 int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_t cout, int32_t chunk);
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
                      int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream);
