@@ -31,65 +31,78 @@ struct Grid3 {
     int D, H, W;
 };
 
-// voxel m of a D x H x W grid shifted by tap (0..26, x fastest: dx = tap % 3 - 1, ...)
-__device__ __forceinline__ int shifted(const Grid3& g, int m, int tap) {
-    const int x = m % g.W, y = (m / g.W) % g.H, z = m / (g.W * g.H);
-    const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
-    if (xx < 0 || yy < 0 || zz < 0 || xx >= g.W || yy >= g.H || zz >= g.D) return -1;
-    return (zz * g.H + yy) * g.W + xx;
-}
+// Taps: 0..26, x fastest (dx = tap % 3 - 1, dy = tap / 3 % 3 - 1, dz = tap / 9 - 1); a tap
+// outside the grid reads zero (zero padding).
 
-// Forward / data-gradient convolution.  One wave: 16 voxels x NT*16 output channels, the
-// 16-channel tiles blockIdx.y * NT .. + NT - 1 (small grids split the channels over blocks).
+// Forward / data-gradient convolution.  One wave: MT x 16 voxels x NT*16 output channels,
+// the 16-channel tiles blockIdx.y * NT .. + NT - 1 (small grids split the channels over
+// blocks); the MT voxel tiles share each weight load.
 // Epilogue: + bias (if given), then either ReLU, or a multiply by the ReLU mask of `gate`
 // (the data gradient of a layer whose input came through a ReLU), stored as bf16 and/or fp32.
-template <int NT>
+template <int NT, int MT>
 __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ X, const __bf16* __restrict__ Wb,
                                                const float* __restrict__ bias, const __bf16* __restrict__ gate,
                                                Grid3 g, int cin, int kp, int cout, int relu,
                                                __bf16* __restrict__ yb, float* __restrict__ yf) {
     const int M = g.D * g.H * g.W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = (blockIdx.x * 4 + wave) * 16;
+    const int m0 = (blockIdx.x * 4 + wave) * 16 * MT;
     if (m0 >= M) return;
     const int r = lane & 15, q = lane >> 4;
-    const int m = m0 + r;
     const int K = 27 * cin;
     const int tb = blockIdx.y * NT;  // first channel tile of this block
-    f32x4 acc[NT];
+    // this lane's voxel of each voxel tile and its coordinates (fixed over the k loop)
+    int mv[MT], xv[MT], yv[MT], zv[MT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < MT; ++u) {
+        mv[u] = m0 + 16 * u + r;
+        xv[u] = mv[u] % g.W;
+        yv[u] = (mv[u] / g.W) % g.H;
+        zv[u] = mv[u] / (g.W * g.H);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < kp; kb += 32) {
         const int k = kb + 8 * q;
-        bf16x8 a;
+        const int tap = k < K ? k / cin : 0, ci = k - tap * cin;
+        const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
+        bf16x8 b[NT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.f;
-        if (k < K && m < M) {
-            const int tap = k / cin, ci = k - tap * cin;
-            const int s = shifted(g, m, tap);
-            if (s >= 0) a = *reinterpret_cast<const bf16x8*>(X + (int64_t)s * cin + ci);
+        for (int t = 0; t < NT; ++t)
+            b[t] = *reinterpret_cast<const bf16x8*>(Wb + (int64_t)((tb + t) * 16 + r) * kp + k);
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            bf16x8 a;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = (__bf16)0.f;
+            const int xx = xv[u] + ox, yy = yv[u] + oy, zz = zv[u] + oz;
+            if (k < K && mv[u] < M && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.W && yy < g.H && zz < g.D)
+                a = *reinterpret_cast<const bf16x8*>(X + (int64_t)((zz * g.H + yy) * g.W + xx) * cin + ci);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[u][t], 0, 0, 0);
         }
+    }
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wb + (int64_t)((tb + t) * 16 + r) * kp + k);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-        }
-    }
+            const int n = (tb + t) * 16 + r;
+            if (n >= cout) continue;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int n = (tb + t) * 16 + r;
-        if (n >= cout) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int mm = m0 + 4 * q + j;
-            if (mm >= M) continue;
-            float v = acc[t][j] + (bias ? bias[n] : 0.f);
-            if (relu) v = v > 0.f ? v : 0.f;
-            if (gate && !(bf2f(gate[(int64_t)mm * cout + n]) > 0.f)) v = 0.f;
-            if (yb) yb[(int64_t)mm * cout + n] = (__bf16)v;
-            if (yf) yf[(int64_t)mm * cout + n] = v;
+            for (int j = 0; j < 4; ++j) {
+                const int mm = m0 + 16 * u + 4 * q + j;
+                if (mm >= M) continue;
+                float v = acc[u][t][j] + (bias ? bias[n] : 0.f);
+                if (relu) v = v > 0.f ? v : 0.f;
+                if (gate && !(bf2f(gate[(int64_t)mm * cout + n]) > 0.f)) v = 0.f;
+                if (yb) yb[(int64_t)mm * cout + n] = (__bf16)v;
+                if (yf) yf[(int64_t)mm * cout + n] = v;
+            }
         }
-    }
 }
 
 // Weight gradient partials.  Block (kt, nt, chunk): one wave per 16 x 16·KT block of (output
@@ -382,20 +395,28 @@ int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate
     __bf16* Y = (__bf16*)yb;
     if (vblocks < 256) {
         // fewer voxel blocks than CUs: one channel tile per block, the tiles over grid.y
-        k_conv3<1><<<dim3(vblocks, ntw), 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+        k_conv3<1, 1><<<dim3(vblocks, ntw), 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     const unsigned grid = vblocks;
     switch (ntw) {
-        case 1: k_conv3<1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 2: k_conv3<2><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 3: k_conv3<3><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 4: k_conv3<4><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 5: k_conv3<5><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 6: k_conv3<6><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 7: k_conv3<7><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        case 8: k_conv3<8><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
-        default: k_conv3<12><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        // one or two channel tiles on grids of 65536 voxels and more: two voxel tiles per wave
+        // share each weight load (on smaller grids the halved block count costs more)
+        case 1:
+            if (M >= 65536) k_conv3<1, 2><<<nblk(M, 128), 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+            else k_conv3<1, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+            break;
+        case 2:
+            if (M >= 65536) k_conv3<2, 2><<<nblk(M, 128), 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+            else k_conv3<2, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf);
+            break;
+        case 3: k_conv3<3, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        case 4: k_conv3<4, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        case 5: k_conv3<5, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        case 6: k_conv3<6, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        case 7: k_conv3<7, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        case 8: k_conv3<8, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
+        default: k_conv3<12, 1><<<grid, 256, 0, s>>>(X, Wb, bias, G, g, cin, kp, cout, relu, Y, yf); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
