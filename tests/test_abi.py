@@ -37,3 +37,23 @@ def test_network_header_matches_library_and_bindings():
     missing = [s for s in declared if not hasattr(lib, s)]
     assert not missing, missing
     assert declared == sorted(nn.SIGNATURES)
+
+
+def test_network_argument_checks_reject_before_any_launch():
+    """The convolution entry points validate their sizes and operand alignment on the host and
+    return -1 without touching the device, so these calls are safe without a GPU."""
+    from mflbm import nn
+    L = nn.lib()
+    buf = ctypes.create_string_buffer(64)
+    aligned = (ctypes.addressof(buf) + 15) // 16 * 16
+    # weight gradient: cout not a multiple of 8, cin not a multiple of 8, kp too small,
+    # chunk not a multiple of 32, misaligned operands
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, 4, 8, 224, 12, 32, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, 4, 12, 352, 8, 32, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, 4, 8, 192, 8, 32, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, 4, 8, 224, 8, 48, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned + 2, aligned, 4, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned, aligned + 8, 4, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
+    # forward: more than 192 output channels, or an empty grid
+    assert L.mfnn_conv3(aligned, aligned, None, None, 4, 4, 4, 8, 224, 200, 1, aligned, None, None) == -1
+    assert L.mfnn_conv3(aligned, aligned, None, None, 0, 4, 4, 8, 224, 16, 1, aligned, None, None) == -1
