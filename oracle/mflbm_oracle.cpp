@@ -18,8 +18,12 @@
 //    AA streaming   : two steps with zero-force rest populations vs explicit pull stream
 //    gradient chain : ISO4 exact on linear/quadratic fields, sphere curvature ~ 2/R,
 //                     wetting angle n.s_n = cos(theta') after the secant iteration
-//    geometry       : brute-force walls_type classification, smoothing mass conservation,
-//                     ISO8 exact on polynomials, plane-wall normals
+//    geometry       : brute-force walls_type classification; the smoothed indicator equals
+//                     four passes of the separable (1,4,1)/6 filter along x, y and z; the
+//                     solid normals equal an ISO8 gradient built by enumerating the
+//                     integer offsets (reference form, linear gain 37/42); flat-wall
+//                     normal length |nw|/(|nw|+FLT_EPSILON) in exact rationals; 45-degree
+//                     slab and sphere normal directions
 //    Zou-He BCs     : imposed per-colour density and zero transverse momentum
 //    periodic copies: periodic run == run on a wrapped domain
 //    monitor        : brute-force sums in numpy
@@ -168,6 +172,7 @@ struct OracleSim {
     std::vector<int32_t> walls;       // s2, 0 fluid / 1 solid
     std::vector<int32_t> walls_type;  // s4, -1 fluid boundary, 0 fluid, 1 solid, 2 solid boundary
     std::vector<double> s_nx, s_ny, s_nz;  // s4 solid normals at fluid-boundary nodes
+    std::vector<double> smooth;       // s4: the smoothed solid indicator the solid normals are taken of
     std::vector<double> pdf;          // 38 * s1
     std::vector<double> phi;          // s4
     std::vector<double> cn_x, cn_y, cn_z, c_norm;  // s2
@@ -287,6 +292,11 @@ void preprocess_geometry(OracleSim& S, const int8_t* geo) {
             for (int64_t j = 2 - G; j <= ny + G - 1; ++j)
                 for (int64_t i = 2 - G; i <= nx + G - 1; ++i) sm1[ti(i, j, k)] = sm2[ti(i, j, k)];
     }
+
+    S.smooth.assign(S.S4, 0.0);
+    for (int64_t k = -3; k <= nz + 4; ++k)
+        for (int64_t j = -3; j <= ny + 4; ++j)
+            for (int64_t i = -3; i <= nx + 4; ++i) S.smooth[S.s4(i, j, k)] = sm2[ti(i, j, k)];
 
     // solid normals at fluid-boundary nodes (src/Geometry_preprocessing.cpp:207-386)
 #pragma omp parallel for collapse(2) schedule(static)
@@ -1174,6 +1184,7 @@ void* oracle_array(void* h, int which, int64_t* count) {
         case 9: v = &S.s_nz; break;
         case 10: v = &S.w_in; break;
         case 11: v = &S.conv; break;
+        case 12: v = &S.smooth; break;
         default: return nullptr;
     }
     if (count) *count = (int64_t)v->size();

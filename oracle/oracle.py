@@ -89,7 +89,7 @@ def _dp(a):
 
 
 ARRAYS = {"pdf": 0, "phi": 1, "cn_x": 2, "cn_y": 3, "cn_z": 4, "c_norm": 5, "curv": 6, "s_nx": 7, "s_ny": 8, "s_nz": 9,
-          "w_in": 10, "convec": 11}
+          "w_in": 10, "convec": 11, "smooth": 12}
 
 
 class Oracle:
@@ -130,7 +130,7 @@ class Oracle:
             return a.reshape(self.ny + 2, self.nx + 2)
         if name == "convec":
             return a.reshape(39, self.ny + 2, self.nx + 2)
-        halo = {"pdf": 1, "phi": 4, "curv": 1, "s_nx": 4, "s_ny": 4, "s_nz": 4}.get(name, 2)
+        halo = {"pdf": 1, "phi": 4, "curv": 1, "s_nx": 4, "s_ny": 4, "s_nz": 4, "smooth": 4}.get(name, 2)
         if name == "pdf":
             return a.reshape((2, 19) + self.shape(1))
         return a.reshape(self.shape(halo))
