@@ -1,0 +1,134 @@
+"""GPU parity at the workloads' real sizes, and on the reference's shipped geometry.
+
+* 256^3 drainage (spheres r = 8, porosity 0.35, theta 30, viscosity ratio 0.1): FP64 bitwise
+  against the oracle after 10 steps; FP32 against FP64 after 200 steps (saturation, each
+  colour's mass, the axial flux);
+* 512 x 256 x 256 Gaussian-random-field drainage (porosity 0.30, correlation length 10):
+  FP64 bitwise against the oracle after 4 steps; FP32 mass drift against FP64 over 300 steps;
+* input/geometry/tube_sphere.dat (60 x 60 x 80, committed under tests/golden/) with the
+  physics of the reference's example control file: 50 FP64 steps bitwise.
+
+The oracle runs with all host threads (a step of 256^3 takes about 0.3 s on 16 cores).
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_same
+from mflbm import cases, config, geometry, gpu
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+T0 = time.time()
+
+
+def say(msg):
+    print(f"[gpu-fullsize {time.time() - T0:7.1f}s] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def compare_bitwise(o, g, tag):
+    for name in ("pdf", "phi"):
+        a = g.download(name)
+        assert_same(a, o.array(name), what=f"{tag} {name}")
+        del a
+    for name in ("cn_x", "cn_y", "cn_z", "c_norm"):
+        assert_same(g.download(name), o.array(name), what=f"{tag} {name}")
+
+
+def compare_geometry(o, g, tag):
+    assert_same(g.download("walls_type").astype(np.int32), o.int_array("walls_type"), what=f"{tag} walls_type")
+    for name in ("s_nx", "s_ny", "s_nz"):
+        assert_same(g.download(name), o.array(name), what=f"{tag} {name}")
+
+
+def saturation(m):
+    return m["vol1"].sum() / (m["vol1"].sum() + m["vol2"].sum())
+
+
+def run_bitwise(geo, p, steps, tag):
+    say(f"case={tag} shape={geo.shape} oracle set-up ({O.threads()} threads)")
+    o = O.Oracle(p, geo)
+    say(f"case={tag} device set-up")
+    g = gpu.Solver(p, geo, gpu.FP64)
+    compare_geometry(o, g, tag)
+    o.setup()
+    g.setup()
+    say(f"case={tag} {steps} steps")
+    o.run(1, steps)
+    g.run(1, steps)
+    say(f"case={tag} compare")
+    compare_bitwise(o, g, f"{tag} step{steps}")
+    o.close()
+    return g
+
+
+def fp32_against_fp64(g64, geo, p, steps, tag):
+    """Runs an FP32 solver and the (already bit-checked) FP64 solver g64 to `steps`;
+    returns the relative differences of saturation, total mass, each colour's mass and
+    the fluid-1 axial flux."""
+    g32 = gpu.Solver(p, geo, gpu.FP32)
+    g32.setup()
+    g32.run(1, steps)
+    m32, m64 = g32.monitor(), g64.monitor()
+    out = {
+        "saturation": abs(saturation(m32) - saturation(m64)),
+        "mass": abs((m32["mass1"] + m32["mass2"]).sum() / (m64["mass1"] + m64["mass2"]).sum() - 1.0),
+        "mass1": abs(m32["mass1"].sum() / m64["mass1"].sum() - 1.0),
+        "mass2": abs(m32["mass2"].sum() / m64["mass2"].sum() - 1.0),
+        "flux1": abs(m32["fl1"].sum() / m64["fl1"].sum() - 1.0),
+    }
+    say(f"case={tag} fp32 vs fp64 after {steps} steps: " + ", ".join(f"{k} {v:.2e}" for k, v in out.items()))
+    g32.close()
+    return out
+
+
+def test_drainage256_fp64_bitwise_and_fp32_tracks():
+    geo, ctrl = cases.drainage256()
+    p = config.derive(ctrl)
+    g = run_bitwise(geo, p, 10, "drainage256")
+    g.run(11, 190)
+    d = fp32_against_fp64(g, geo, p, 200, "drainage256")
+    # measured: saturation 2e-9, mass 3e-9, colour masses 1e-8, flux 1e-5
+    assert d["saturation"] < 1e-6
+    assert d["mass"] < 1e-7
+    assert d["mass1"] < 1e-6 and d["mass2"] < 1e-6
+    assert d["flux1"] < 1e-3
+    g.close()
+
+
+def test_grf512_fp64_bitwise_and_fp32_mass_drift():
+    geo, ctrl = cases.grf_drainage512()
+    p = config.derive(ctrl)
+    g = run_bitwise(geo, p, 4, "grf_drainage512")
+    g.run(5, 296)
+    d = fp32_against_fp64(g, geo, p, 300, "grf_drainage512")
+    assert d["saturation"] < 1e-6
+    assert d["mass"] < 1e-7
+    assert d["mass1"] < 1e-6 and d["mass2"] < 1e-6
+    g.close()
+
+
+def tube_sphere_case():
+    """The reference's example control (input/simulation_control.txt: imbibition, option 2,
+    fluid 2 injected through a Zou-He pressure inlet, walls on x and y, viscosities 0.04 /
+    0.4, surface tension 0.03, theta 65, porous plate blocking fluid 2) on its shipped
+    geometry.  The domain is sized to the geometry and the plate kept 8 planes from the
+    outlet, as at Z_porous_plate 252 of nz 260."""
+    with open(os.path.join(GOLDEN, "reference_control.json")) as fh:
+        ctrl = json.load(fh)
+    geo = geometry.read_dat(os.path.join(GOLDEN, "tube_sphere.dat"))
+    nz, ny, nx = geo.shape
+    ctrl.update(nxGlobal=nx, nyGlobal=ny, nzGlobal=nz, Z_porous_plate=nz - 8)
+    return geo, config.derive(ctrl)
+
+
+def test_tube_sphere_reference_geometry_bitwise():
+    geo, p = tube_sphere_case()
+    g = run_bitwise(geo, p, 50, "tube_sphere")
+    g.close()
