@@ -54,23 +54,50 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(geo, params, steps):
-    """Time the OpenMP FP64 oracle on this host over `steps` steps of the same workload."""
+def cpu_baseline_child(args):
+    """Child process of the CPU baseline: times the OpenMP FP64 oracle on this host over
+    `--cpu-steps` steps of the same workload (the reference's timing: steady clock around
+    the step loop only, src/main.cpp:142-199) and prints one JSON line."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O  # the oracle doubles as the reported CPU baseline
 
+    geo = np.load(args.cpu_child, allow_pickle=False)
+    n = geo.shape[0]
+    _, ctrl = cases.drainage256(n=(n, n, n), geometry=False)
+    params = config.derive(ctrl)
     threads = O.threads()
     o = O.Oracle(params, geo)
     o.setup()
-    o.run(1, 2)  # warm caches / first-touch
+    o.run(1, 2)  # first touch of every page by the threads that own it
     t0 = time.perf_counter()
-    o.run(3, steps)
+    o.run(3, args.cpu_steps)
     dt = time.perf_counter() - t0
-    nodes = params["nx"] * params["ny"] * params["nz"]
     o.close()
-    return {"value": nodes * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same {params['nx']}x{params['ny']}x{params['nz']} workload, "
-                      f"OpenMP FP64 oracle, {threads} threads on {cpu_model()}"}
+    print(json.dumps({"seconds": dt, "threads": threads}), flush=True)
+
+
+def cpu_baseline(geo, steps):
+    """The CPU baseline in a child process whose OpenMP runtime starts with the placement
+    the reference's plan states: OMP_PROC_BIND=close, OMP_PLACES=cores, all host threads
+    (OMP_NUM_THREADS as set on the host)."""
+    tmp = tempfile.mkdtemp(prefix="mflbm_cpu_", dir="/tmp")
+    geo_path = os.path.join(tmp, "geo.npy")
+    np.save(geo_path, geo)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", geo_path,
+                            "--cpu-steps", str(steps)], env=env, capture_output=True, text=True, timeout=600)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed (rc {r.returncode}): {r.stderr[-400:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    n = geo.shape[0]
+    nodes = geo.size
+    return {"value": nodes * steps / res["seconds"] / 1e6, "unit": "MLUPS", "cores": res["threads"], "kind": "port",
+            "seconds": round(res["seconds"], 2),
+            "sample": f"{steps} steps (3 .. {steps + 2}) of the same {n}x{n}x{n} workload, OpenMP FP64 oracle, "
+                      f"{res['threads']} threads (OMP_PROC_BIND=close, OMP_PLACES=cores) on {cpu_model()}"}
 
 
 PROBE_KERNEL = "k_probe_copy"
@@ -198,7 +225,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--n", type=int, default=256, help="cubic lattice edge")
-    ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="steps of the CPU baseline sample, about 30 s on 16 host cores at 256^3 (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--late-steps", type=int, default=4000,
                     help="further drainage steps before a second, later measurement (0 = skip)")
@@ -206,9 +234,13 @@ def main():
     ap.add_argument("--traffic", action=argparse.BooleanOptionalAction, default=True,
                     help="measure HBM bytes per kernel with two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
+        return
+    if args.cpu_child:
+        cpu_baseline_child(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU z-slab "
@@ -258,11 +290,13 @@ def main():
     bnd_ms = kt["boundary_ms"] / n_steps
     step_ms = coll_ms + bnd_ms + grad_ms
     # algorithmic bytes of one collide launch: 38 slots read + 38 written + phi written per
-    # fluid node, plus the 4-value force record read per fluid node
-    coll_bytes = n_fluid * (76 * s + s + 4 * s)
+    # fluid node (77 s); the force records it reads are derivable from phi and not counted
+    coll_bytes = n_fluid * 77 * s
     achieved = coll_bytes / (coll_ms * 1e-3) / 1e9
-    # step: collide + gradient chain (phi read and force records written per fluid node)
-    step_bytes = coll_bytes + n_fluid * (s + 4 * s)
+    # one step: 76 s of populations + phi written once and read once per fluid node, plus
+    # one byte of node type per lattice node, over the timed wall-clock step
+    step_bytes = n_fluid * 78 * s + n_all
+    step_frac = step_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS
     log(f"collide odd {odd_ms:.3f} ms, even {even_ms:.3f} ms ({achieved:.0f} GB/s algorithmic), "
         f"boundary {bnd_ms:.3f} ms, gradient {grad_ms:.3f} ms, step (kernels) {step_ms:.3f} ms")
 
@@ -316,8 +350,8 @@ def main():
     cpu = None
     if args.cpu_steps > 0:
         log(f"CPU baseline: {args.cpu_steps} oracle steps")
-        cpu = cpu_baseline(geo, params, args.cpu_steps)
-        log(f"CPU baseline {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
+        cpu = cpu_baseline(geo, args.cpu_steps)
+        log(f"CPU baseline {cpu['value']:.2f} MLUPS on {cpu['cores']} threads ({cpu['seconds']} s)")
 
     coll_traffic = None
     if traffic and "collide_odd" in traffic["requests"] and "collide_even" in traffic["requests"]:
@@ -336,7 +370,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": coll_traffic,
                      "kernel": "k_collide (mean of the odd and even launches)",
                      "algorithmic_bytes": int(coll_bytes)},
-        "step_roofline_frac": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_roofline": {"bytes": int(step_bytes), "ms": round(dt / args.steps * 1e3, 4),
+                          "frac": round(step_frac, 4),
+                          "definition": "(N_fluid * 78 * s + N_all * 1 B) / timed ms_per_step / 8 TB/s"},
+        "fluid_glups": round(n_fluid * args.steps / dt / 1e9, 4),
         "kernel_ms": {"collide_odd": round(odd_ms, 4), "collide_even": round(even_ms, 4),
                       "boundary": round(bnd_ms, 4), "gradient": round(grad_ms, 4), "step": round(step_ms, 4)},
         "late": late,

@@ -33,7 +33,8 @@ int mfnn_sync(void);
 
 /* y = conv3x3x3(x, w) + bias, then ReLU (relu != 0) or a multiply by [gate > 0] (gate != null,
  * gate has cout channels); stored to yb (bf16) and/or yf (fp32, either may be null).
- * cin % 8 == 0, cout <= 192 (w has 16 * ceil(cout / 16) rows, or 192 when cout > 128).
+ * cin % 8 == 0, cout <= 192 (w has 16 * ceil(cout / 16) rows, or 192 when cout > 128),
+ * D, H, W > 0, x and w 16-byte aligned (returns -1 otherwise).
  * With w from mfnn_flip_weights this is the data gradient. */
 int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate, int32_t D, int32_t H, int32_t W,
                int32_t cin, int32_t kp, int32_t cout, int32_t relu, void* yb, float* yf, void* stream);
@@ -41,7 +42,8 @@ int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate
 /* dw[n][k] (+)= sum_voxels dz[m][n] * x[m + off(tap)][ci] and db[n] (+)= sum_m dz[m][n]
  * (accumulate != 0 adds to the existing values).  The voxel sum runs in chunks of `chunk`
  * voxels (multiple of 32) whose fp32 partials go to `work` (mfnn_wgrad_workspace floats).
- * cin and cout are multiples of 8 and x, dz 16-byte aligned (returns -1 otherwise). */
+ * cin and cout are multiples of 8, D, H, W > 0 and x, dz 16-byte aligned (returns -1
+ * otherwise). */
 int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_t cout, int32_t chunk);
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
                      int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream);

@@ -383,6 +383,7 @@ int mfnn_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -2; }
 int mfnn_conv3(const void* x, const void* w, const float* bias, const void* gate, int32_t D, int32_t H, int32_t W,
                int32_t cin, int32_t kp, int32_t cout, int32_t relu, void* yb, float* yf, void* stream) {
     if (cin % 8 || kp % 32 || kp < 27 * cin || cout <= 0 || cout > 192 || D <= 0 || H <= 0 || W <= 0) return -1;
+    if ((uintptr_t)x % 16 || (uintptr_t)w % 16) return -1;  // 16-byte operand loads
     const Grid3 g{D, H, W};
     const int64_t M = (int64_t)D * H * W;
     const unsigned vblocks = nblk(M, 64);
@@ -429,7 +430,8 @@ int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_
 
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
                      int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream) {
-    if (cin % 8 || cout % 8 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return -1;
+    if (cin % 8 || cout % 8 || cin <= 0 || cout <= 0 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return -1;
+    if (D <= 0 || H <= 0 || W <= 0) return -1;
     if ((uintptr_t)x % 16 || (uintptr_t)dz % 16) return -1;  // 16-byte operand loads
     const Grid3 g{D, H, W};
     const int64_t M = (int64_t)D * H * W;

@@ -54,6 +54,11 @@ def test_network_argument_checks_reject_before_any_launch():
     assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, 4, 8, 224, 8, 48, None, None, None, 0, None) == -1
     assert L.mfnn_conv3_wgrad(aligned + 2, aligned, 4, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
     assert L.mfnn_conv3_wgrad(aligned, aligned + 8, 4, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
-    # forward: more than 192 output channels, or an empty grid
+    # weight gradient: an empty grid
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 0, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, -1, 8, 224, 8, 32, None, None, None, 0, None) == -1
+    # forward: more than 192 output channels, an empty grid, misaligned input or weights
     assert L.mfnn_conv3(aligned, aligned, None, None, 4, 4, 4, 8, 224, 200, 1, aligned, None, None) == -1
     assert L.mfnn_conv3(aligned, aligned, None, None, 0, 4, 4, 8, 224, 16, 1, aligned, None, None) == -1
+    assert L.mfnn_conv3(aligned + 2, aligned, None, None, 4, 4, 4, 8, 224, 16, 1, aligned, None, None) == -1
+    assert L.mfnn_conv3(aligned, aligned + 8, None, None, 4, 4, 4, 8, 224, 16, 1, aligned, None, None) == -1

@@ -39,8 +39,12 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-@pytest.mark.parametrize("cin,cout,shape", [(8, 16, (6, 5, 7)), (16, 8, (8, 8, 8)), (24, 40, (4, 6, 8)),
-                                            (48, 192, (4, 4, 4))])
+@pytest.mark.parametrize("cin,cout,shape", [
+    (8, 16, (6, 5, 7)), (16, 8, (8, 8, 8)), (24, 40, (4, 6, 8)), (48, 192, (4, 4, 4)),
+    # the large-grid launches: 65536 voxels (two voxel tiles per wave with two channel tiles,
+    # weight gradient in chunks of 2048), 18432 voxels (three channel tiles over the voxel
+    # grid), 16384 voxels with 144 channels (twelve tiles, 192 weight rows)
+    (8, 32, (32, 32, 64)), (16, 48, (24, 24, 32)), (16, 144, (16, 32, 32))])
 def test_conv_forward_dgrad_wgrad(nn, cin, cout, shape):
     say(f"conv3 cin={cin} cout={cout} shape={shape}")
     rng = np.random.default_rng(cin * 100 + cout)
