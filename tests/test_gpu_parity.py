@@ -242,6 +242,13 @@ def test_fp32_tracks_fp64():
     m_ref = ref["mass1"].sum() + ref["mass2"].sum()
     m_got = got["mass1"].sum() + got["mass2"].sum()
     assert abs(m_got - m_ref) / m_ref < 1e-4
+    # each colour's mass on its own: the FP32 reading of a colour-2 trace must neither
+    # create nor remove fluid 2
+    d1 = abs(got["mass1"].sum() / ref["mass1"].sum() - 1.0)
+    d2 = abs(got["mass2"].sum() / ref["mass2"].sum() - 1.0)
+    say(f"fp32 vs fp64 oracle after 200 steps: saturation {abs(s_got - s_ref):.2e}, mass {abs(m_got / m_ref - 1):.2e}, "
+        f"mass1 {d1:.2e}, mass2 {d2:.2e}")
+    assert d1 < 1e-4 and d2 < 1e-4, (d1, d2)
     assert rel_linf(g.download("phi"), o.array("phi")) < 0.05
 
 
