@@ -56,17 +56,25 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     constexpr int SEG = (NP + GRAD_WAVES - 1) / GRAD_WAVES;        // entries per wave
     constexpr int NSUB = (SEG + 63) / 64;                          // entries per lane
     static_assert(NP <= 1024, "list entries keep the plane position in 10 bits");
+    constexpr int N3 = (TX + 6) * (TY + 6), N4L = NV, N6 = (TX + 2) * (TY + 2);  // list capacities
     __shared__ T s_phi[4][NP];
     __shared__ T s_cn[4][3][NV];
     __shared__ T s_cnorm[3][NO];
-    // work lists of the four stages, double-buffered by iteration parity: loaded-plane
-    // entries that a stage computes, one segment per wave.  K3 and K6 entries carry the
-    // fluid-neighbour mask (e | mask << 10), K4 entries the wetting flag (e | wet << 14).
-    __shared__ int32_t s_lbnd[2][2][GRAD_WAVES * SEG];  // K3, K6
-    __shared__ int16_t s_lfl[2][2][GRAD_WAVES * SEG];   // K4, K7
-    __shared__ int s_cnt[2][4][GRAD_WAVES];
+    // work lists of the four stages, double-buffered by iteration parity: the loaded-plane
+    // entries a stage computes, packed in one run per list (each wave reserves its place
+    // with one LDS atomic on the list's count).  K3 and K6 entries carry the fluid-neighbour
+    // mask (e | mask << 10), K4 entries the wetting flag (e | wet << 14), K7 entries come
+    // with the node's force-record index.
+    __shared__ int32_t s_l3[2][N3], s_l6[2][N6];
+    __shared__ int16_t s_l4[2][N4L], s_l7[2][NO];
+    __shared__ int32_t s_r7[2][NO];
+    __shared__ unsigned long long s_cnt[2];  // list counts (16 bits each: K3, K4, K6, K7)
     // 1 when K4 found a non-zero normal anywhere in the normals region of the plane
     __shared__ int s_any[4];
+    // K3/K6 weight sums: the D3Q19 weights of the fluid neighbours summed in q order are the
+    // 1/18 of the a axis neighbours added a times, then the 1/36 of the d diagonal ones d
+    // times, so the sum depends on (a, d) only; s_ws[a][d] holds it, summed that way
+    __shared__ T s_ws[7][13];
 
     const Dims& d = A.d;
     const int b = xcd_block();
@@ -113,24 +121,21 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int sb = 0; sb < NSUB; ++sb) th[a][sb] = TAG_SOLID;
-    // i-th entry of a stage list (segments concatenated in wave order), clamped into the
-    // list's value range so that no address formed from it can leave the arrays (callers
-    // only ask for i below the list total)
-    auto item16 = [&](const int16_t* list, const int* cnt, int i) -> int {
-        const int v = list_item<GRAD_WAVES>(list, cnt, SEG, i);
-        return v < 0 ? 0 : v;
-    };
-    auto item32 = [&](const int32_t* list, const int* cnt, int i) -> int {
-        const int v = list_item<GRAD_WAVES>(list, cnt, SEG, i);
-        return v < 0 ? 0 : v;
-    };
+    // plane position of a list entry, clamped so that no address formed from it can leave
+    // the arrays
     auto pos = [](int v) { return min(v & 1023, NP - 1); };
-    auto total = [](const int* cnt) {
-        int t = 0;
-#pragma unroll
-        for (int w = 0; w < GRAD_WAVES; ++w) t += cnt[w];
-        return t;
-    };
+    auto count = [&](int pr, int l) { return (int)((s_cnt[pr] >> (16 * l)) & 0xFFFF); };
+    auto wsum = [&](uint32_t mask) -> T { return s_ws[__popc(mask & 63u)][__popc(mask >> 6)]; };
+
+    if (tid < 7 * 13) {
+        const int a = tid / 13, dd = tid % 13;
+        T ws = T(0);
+        for (int m = 0; m < a; ++m) ws += W<T>::w1;
+        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
+        s_ws[a][dd] = ws;
+    }
+    if (tid < 2) s_cnt[tid] = 0;
+    __syncthreads();
 
     fetch(k0 - H);
     int32_t tcur[NSUB];
@@ -159,15 +164,12 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         // the normals slot K4 fills in this iteration held plane p-6, which no stage reads
         // any more; its flag starts over
         if (tid == 0) s_any[s4(p - 2)] = 0;
-        // stage lists of this iteration, from the tags of earlier planes kept in registers
+        // stage lists of this iteration, from the tags of earlier planes kept in registers:
+        // each wave counts its entries, reserves a run of each list with one LDS atomic on
+        // the four packed counts and writes its entries there (the order within a list does
+        // not matter: every entry is computed on its own)
         {
-            int c3 = 0, c4 = 0, c6 = 0, c7 = 0;
-            int32_t* l3 = &s_lbnd[par][0][wave * SEG];
-            int32_t* l6 = &s_lbnd[par][1][wave * SEG];
-            int16_t* l4 = &s_lfl[par][0][wave * SEG];
-            int16_t* l7 = &s_lfl[par][1][wave * SEG];
-#pragma unroll
-            for (int sb = 0; sb < NSUB; ++sb) {
+            auto flags = [&](int sb, bool& f3, bool& f4, bool& f6, bool& f7) {
                 bool in;
                 const int e = entry(sb, in);
                 const int x = e % WX, y = e / WX;
@@ -175,28 +177,49 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 const bool r2 = in && x >= 2 && x < WX - 2 && y >= 2 && y < WY - 2;
                 const bool r1 = in && x >= 3 && x < WX - 3 && y >= 3 && y < WY - 3;
                 const bool r0 = in && x >= H && x < H + nxo && y >= H && y < H + nyo;
-                const int t3 = th[0][sb], t4 = th[1][sb], t6 = th[2][sb], t7 = th[3][sb];
-                const bool f3 = r3 && (t3 & 3) == TAG_SOLID_BND;
-                const bool f4 = r2 && tag_fluid(t4 & 3);
-                const bool f6 = r1 && (t6 & 3) == TAG_SOLID_BND;
-                const bool f7 = r0 && tag_fluid(t7 & 3);
-                auto push = [&](bool f, auto* l, int& c, int v) {
-                    const uint64_t m = __ballot(f);
-                    const int at = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                    if (f) l[c + at] = v;
-                    c += __popcll(m);
-                };
-                push(f3, l3, c3, e | ((t3 >> 2) << 10));
-                push(f4, l4, c4, e | (((t4 & 3) == TAG_WET) << 14));
-                push(f6, l6, c6, e | ((t6 >> 2) << 10));
-                push(f7, l7, c7, e);
+                f3 = r3 && (th[0][sb] & 3) == TAG_SOLID_BND;
+                f4 = r2 && tag_fluid(th[1][sb] & 3);
+                f6 = r1 && (th[2][sb] & 3) == TAG_SOLID_BND;
+                f7 = r0 && tag_fluid(th[3][sb] & 3);
+                return e;
+            };
+            uint64_t cnt = 0;  // four 16-bit counts: K3, K4, K6, K7
+#pragma unroll
+            for (int sb = 0; sb < NSUB; ++sb) {
+                bool f3, f4, f6, f7;
+                flags(sb, f3, f4, f6, f7);
+                cnt += (uint64_t)__popcll(__ballot(f3)) | (uint64_t)__popcll(__ballot(f4)) << 16 |
+                       (uint64_t)__popcll(__ballot(f6)) << 32 | (uint64_t)__popcll(__ballot(f7)) << 48;
             }
-            if (lane == 0) {
-                s_cnt[par][0][wave] = c3; s_cnt[par][1][wave] = c4;
-                s_cnt[par][2][wave] = c6; s_cnt[par][3][wave] = c7;
+            uint64_t base = 0;
+            if (lane == 0) base = atomicAdd(&s_cnt[par], (unsigned long long)cnt);
+            base = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+            int b3 = (int)(base & 0xFFFF), b4 = (int)((base >> 16) & 0xFFFF);
+            int b6 = (int)((base >> 32) & 0xFFFF), b7 = (int)(base >> 48);
+            auto below = [&](uint64_t m) {
+                return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            };
+#pragma unroll
+            for (int sb = 0; sb < NSUB; ++sb) {
+                bool f3, f4, f6, f7;
+                const int e = flags(sb, f3, f4, f6, f7);
+                const uint64_t m3 = __ballot(f3), m4 = __ballot(f4), m6 = __ballot(f6), m7 = __ballot(f7);
+                if (f3) s_l3[par][b3 + below(m3)] = e | ((th[0][sb] >> 2) << 10);
+                if (f4) s_l4[par][b4 + below(m4)] = (int16_t)(e | (((th[1][sb] & 3) == TAG_WET) << 14));
+                if (f6) s_l6[par][b6 + below(m6)] = e | ((th[2][sb] >> 2) << 10);
+                if (f7) {
+                    const int at = b7 + below(m7);
+                    s_l7[par][at] = (int16_t)e;
+                    s_r7[par][at] = th[3][sb] >> 2;
+                }
+                b3 += __popcll(m3); b4 += __popcll(m4); b6 += __popcll(m6); b7 += __popcll(m7);
             }
         }
         lds_barrier();
+        // the lists of the other parity were last read before this barrier (by K7 of the
+        // previous iteration); their counts start over for the next iteration
+        if (tid == 0) s_cnt[par ^ 1] = 0;
 
         // Global reads of the K4 and K7 stages, issued now so that their latency hides
         // behind K3: the solid normals of the fluid-boundary nodes of plane p-2 and the
@@ -206,15 +229,14 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         int32_t rec = 0;
         if (p - 2 >= k0 - 2) {
             const int q = p - 2;
-            const int16_t* list = s_lfl[par][0];
-            const int* cnt = s_cnt[par][1];
-            const int n = total(cnt);
+            const int16_t* list = s_l4[par];
+            const int n = min(count(par, 1), N4L);
 #pragma unroll
             for (int m = 0; m < N4; ++m) {
                 const int i = tid + m * GRAD_THREADS;
                 wn[m][0] = wn[m][1] = wn[m][2] = T(0);
                 if (i < n) {
-                    const int v = item16(list, cnt, i), e = pos(v);
+                    const int v = list[i], e = pos(v);
                     if (v & (1 << 14)) {
                         const int c = i4(d, ib + e % WX, jb + e / WX, q);
                         wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
@@ -222,14 +244,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 }
             }
         }
-        if (p - 4 >= k0) {
-            const int* cnt = s_cnt[par][3];
-            if (tid < total(cnt)) {
-                const int e = pos(item16(s_lfl[par][1], cnt, tid)), x = e % WX, y = e / WX;
-                if (x >= H && x < H + nxo && y >= H && y < H + nyo)  // owned (the list holds only such)
-                    rec = A.tag[i4(d, ib + x, jb + y, p - 4)] >> 2;
-            }
-        }
+        if (p - 4 >= k0 && tid < min(count(par, 3), NO)) rec = s_r7[par][tid];
 
         // K3: phi at solid-boundary nodes of plane q (src/main_iteration_GPU.cu:732-755)
         if (p - 1 >= k0 - 3) {
@@ -237,23 +252,21 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             T* pc = s_phi[s4(q)];
             const T* pl[3] = {s_phi[s4(q - 1)], pc, s_phi[s4(q + 1)]};
             const bool wb = q >= kw0 && q <= kw1;
-            const int32_t* list = s_lbnd[par][0];
-            const int* cnt = s_cnt[par][0];
-            const int n = total(cnt);
+            const int32_t* list = s_l3[par];
+            const int n = min(count(par, 0), N3);
             for (int i = tid; i < n; i += GRAD_THREADS) {
-                const int lv = item32(list, cnt, i), e = pos(lv), x = e % WX, y = e / WX;
+                const int lv = list[i], e = pos(lv), x = e % WX, y = e / WX;
                 const uint32_t mask = (uint32_t)lv >> 10;
-                // branch-free sums: a non-fluid neighbour adds +0 * w = +0, which leaves
-                // the (never negative-zero) partial sums unchanged
-                T ps = T(0), ws = T(0);
+                // branch-free sum: a non-fluid neighbour adds +0 * w = +0, which leaves the
+                // (never negative-zero) partial sum unchanged
+                T ps = T(0);
 #pragma unroll
                 for (int qq = 1; qq < 19; ++qq) {
                     const int o = e + EX[qq] + WX * EY[qq];
                     const bool f = (mask >> (qq - 1)) & 1u;
                     ps += keep_if(f, pl[1 + EZ[qq]][o]) * wq<T>(qq);
-                    ws += keep_if(f, wq<T>(qq));
                 }
-                const T v = ps / ws;
+                const T v = ps / wsum(mask);
                 pc[e] = v;
                 if (wb && x >= H && x < H + nxo && y >= H && y < H + nyo) A.phi[i4(d, ib + x, jb + y, q)] = v;
             }
@@ -266,14 +279,13 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const T* pl[3] = {s_phi[s4(q - 1)], s_phi[s4(q)], s_phi[s4(q + 1)]};
             T* cn = &s_cn[s4(q)][0][0];
             T* cno = s_cnorm[s3(q)];
-            const int16_t* list = s_lfl[par][0];
-            const int* cnt = s_cnt[par][1];
-            const int n = total(cnt);
+            const int16_t* list = s_l4[par];
+            const int n = min(count(par, 1), N4L);
 #pragma unroll
             for (int m = 0; m < N4; ++m) {
                 const int i = tid + m * GRAD_THREADS;
                 if (i >= n) break;
-                const int v = item16(list, cnt, i), e = pos(v), x = e % WX, y = e / WX;
+                const int v = list[i], e = pos(v), x = e % WX, y = e / WX;
                 const int r = (x - 2) + VX * (y - 2);
                 auto a = [&](int dx, int dy, int dz) -> T { return pl[1 + dz][e + dx + WX * dy]; };
                 const T cx = iso4_at<T, 0>(a);
@@ -296,20 +308,19 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int q = p - 3;
             const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
             T* cn = &s_cn[s4(q)][0][0];
-            const int32_t* list = s_lbnd[par][1];
-            const int* cnt = s_cnt[par][2];
-            const int n = total(cnt);
+            const int32_t* list = s_l6[par];
+            const int n = min(count(par, 2), N6);
             // with no interface normal in planes q-1 .. q+1 every average below is +0 / ws
             const bool any = (s_any[s4(q - 1)] | s_any[s4(q)] | s_any[s4(q + 1)]) != 0;
             for (int i = tid; i < n; i += GRAD_THREADS) {
-                const int lv = item32(list, cnt, i), e = pos(lv), x = e % WX, y = e / WX;
+                const int lv = list[i], e = pos(lv), x = e % WX, y = e / WX;
                 const uint32_t mask = (uint32_t)lv >> 10;
                 const int v = (x - 2) + VX * (y - 2);
                 if (!any) {
                     cn[v] = T(0); cn[NV + v] = T(0); cn[2 * NV + v] = T(0);
                     continue;
                 }
-                T sx = T(0), sy = T(0), sz = T(0), ws = T(0);
+                T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
                 for (int qq = 1; qq < 19; ++qq) {
                     const bool f = (mask >> (qq - 1)) & 1u;
@@ -318,8 +329,8 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                     sx += keep_if(f, c[o]) * wq<T>(qq);
                     sy += keep_if(f, c[NV + o]) * wq<T>(qq);
                     sz += keep_if(f, c[2 * NV + o]) * wq<T>(qq);
-                    ws += keep_if(f, wq<T>(qq));
                 }
+                const T ws = wsum(mask);
                 cn[v] = sx / ws; cn[NV + v] = sy / ws; cn[2 * NV + v] = sz / ws;
             }
         }
@@ -331,12 +342,11 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
             const int q = p - 4;
             const T* cl[3] = {&s_cn[s4(q - 1)][0][0], &s_cn[s4(q)][0][0], &s_cn[s4(q + 1)][0][0]};
             const T* cno = s_cnorm[s3(q)];
-            const int16_t* list = s_lfl[par][1];
-            const int* cnt = s_cnt[par][3];
-            const int n = total(cnt);
+            const int16_t* list = s_l7[par];
+            const int n = min(count(par, 3), NO);
             static_assert(NO <= GRAD_THREADS, "one owned node per thread");
             if (tid < n) {
-                const int e = pos(item16(list, cnt, tid)), x = e % WX, y = e / WX;
+                const int e = pos(list[tid]), x = e % WX, y = e / WX;
                 const int v = (x - 2) + VX * (y - 2);
                 auto comp = [&](int c) {
                     return [&, c](int dx, int dy, int dz) -> T { return cl[1 + dz][c * NV + v + dx + VX * dy]; };

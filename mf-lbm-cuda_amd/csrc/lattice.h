@@ -147,25 +147,6 @@ constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}, {
 constexpr int GRAD_NSHAPES = sizeof(GRAD_SHAPES) / sizeof(GRAD_SHAPES[0]);
 constexpr int GRAD_SHAPE_DEFAULT = 0, GRAD_SHAPE_FP32 = 3;
 
-// Work lists of the fused gradient kernel are built per wave: wave w appends to segment
-// w (capacity seg) and records its count cnt[w].  Item i of the concatenated list lies in
-// the last segment whose running prefix does not exceed i.  Returns the entry, or -1 for
-// i outside the list.
-template <int WAVES, typename L>
-MFLBM_HD int list_item(const L* list, const int* cnt, int seg, int i) {
-    int sg = 0, base = 0, acc = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES - 1; ++w) {
-        acc += cnt[w];
-        const bool past = i >= acc;
-        sg = past ? w + 1 : sg;
-        base = past ? acc : base;
-    }
-    const int k = i - base;
-    if (i < 0 || k >= cnt[sg]) return -1;
-    return (int)list[sg * seg + k];
-}
-
 constexpr int TAG_FLUID = 0, TAG_SOLID = 1, TAG_SOLID_BND = 2, TAG_WET = 3;
 MFLBM_HD bool tag_fluid(int code) { return code == TAG_FLUID || code == TAG_WET; }
 
