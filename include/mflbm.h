@@ -102,6 +102,14 @@ const char* mflbm_last_error(const mflbm_handle* h);
 /* Initial order parameter of a new run (src/Init_multiphase.cpp:299-358), options 1-5. */
 int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0);
 
+/* Option 6, the random phase distribution (src/Init_multiphase.cpp:344-350): phi = 1, or -1
+ * where the node's uniform deviate exceeds sa_target (target_fluid1_saturation), over
+ * [0, n+1]^3, then the inlet ghost layers as for the other options.  The reference draws
+ * rand() / RAND_MAX seeded from time(NULL), so no two of its runs agree; here each node's
+ * deviate is a counter-based function of `seed` and the node's s1 index (splitmix64), so a
+ * run is reproducible and independent of the launch order. */
+int mflbm_init_phi_random(mflbm_handle* h, double sa_target, uint64_t seed);
+
 /* Equilibrium populations from rho,u,v,w (s1 layout, host memory, double; any may be
  * null = rho 1 / velocity 0) and the current phi (src/Init_multiphase.cpp:389-443).
  * This is the warm-start hook. */

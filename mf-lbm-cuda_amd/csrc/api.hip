@@ -66,6 +66,7 @@ struct mflbm_handle {
     double *mon_part = nullptr, *mon_out = nullptr;
     int mon_nsplit = 0;
     double *vprev = nullptr, *vpart = nullptr;  // mflbm_velocity_change: 3 nf previous velocity, block sums
+    double* macro_buf = nullptr;               // mflbm_macro: one s1 field
     bool vprev_valid = false;
     // captured step graphs: [0] even step, [1] odd step
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -230,13 +231,14 @@ void free_all(mflbm_handle* h) {
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
                     h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
                     h->ext_node, h->ext_off, h->zstart,
-                    h->tag};
+                    h->tag, h->macro_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
     h->vprev = h->vpart = nullptr;
+    h->macro_buf = nullptr;
     h->cells = h->aux = nullptr;
     h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->pmap = h->ext_node = h->ext_off = h->zstart = nullptr;
     h->w_in = h->conv = nullptr;
@@ -573,15 +575,28 @@ void mflbm_destroy(mflbm_handle* h) {
     delete h;
 }
 
-int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0) {
-    if (!h) return -1;
-    if (option < 1 || option > 5)
-        return fail(h, "initial_fluid_distribution_option %d not supported (options 1-5)", option);
-    if (h->prec == MFLBM_FP64) Launch<double>::init_phi(fields<double>(h), option, interface_z0, h->p.phi_inlet, h->open_z, h->stream);
-    else Launch<float>::init_phi(fields<float>(h), option, interface_z0, h->p.phi_inlet, h->open_z, h->stream);
+static int init_phi(mflbm_handle* h, int option, double interface_z0, double sa_target, uint64_t seed) {
+    const double pin = h->p.phi_inlet;
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::init_phi(fields<double>(h), option, interface_z0, pin, h->open_z, sa_target, seed, h->stream);
+    else Launch<float>::init_phi(fields<float>(h), option, interface_z0, pin, h->open_z, sa_target, seed, h->stream);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
+}
+
+int mflbm_init_phi(mflbm_handle* h, int32_t option, double interface_z0) {
+    if (!h) return -1;
+    if (option < 1 || option > 5)
+        return fail(h, "initial_fluid_distribution_option %d not supported by mflbm_init_phi (options 1-5; 6: "
+                       "mflbm_init_phi_random)", option);
+    return init_phi(h, option, interface_z0, 0.0, 0);
+}
+
+int mflbm_init_phi_random(mflbm_handle* h, double sa_target, uint64_t seed) {
+    if (!h) return -1;
+    if (!(sa_target >= 0.0 && sa_target <= 1.0)) return fail(h, "target_fluid1_saturation %g outside [0, 1]", sa_target);
+    return init_phi(h, 6, 0.0, sa_target, seed);
 }
 
 static int init_populations(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w,
@@ -754,22 +769,21 @@ int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w) {
     if (!h) return -1;
     const int64_t n = h->d.S1;
     double* host[4] = {rho, u, v, w};
-    double* dv[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int a = 0; a < 4; ++a)
-        if (host[a]) {
-            HIPCHK(h, hipMalloc(&dv[a], n * sizeof(double)));
-            HIPCHK(h, hipMemsetAsync(dv[a], 0, n * sizeof(double), h->stream));
-        }
-    if (h->prec == MFLBM_FP64) Launch<double>::macro(sparse<double>(h), consts<double>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
-    else Launch<float>::macro(sparse<float>(h), consts<float>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    for (int a = 0; a < 4; ++a)
-        if (dv[a]) {
-            if (e == hipSuccess) e = hipMemcpy(host[a], dv[a], n * sizeof(double), hipMemcpyDeviceToHost);
-            (void)hipFree(dv[a]);
-        }
-    if (e != hipSuccess) return fail(h, "macro failed: %s", hipGetErrorString(e));
+    // one field at a time through one s1 buffer that the handle keeps (allocated on the
+    // first call, freed with the handle)
+    if (!h->macro_buf) HIPCHK(h, hipMalloc(&h->macro_buf, n * sizeof(double)));
+    for (int a = 0; a < 4; ++a) {
+        if (!host[a]) continue;
+        double* dv[4] = {nullptr, nullptr, nullptr, nullptr};
+        dv[a] = h->macro_buf;
+        HIPCHK(h, hipMemsetAsync(h->macro_buf, 0, n * sizeof(double), h->stream));  // solid nodes stay 0
+        if (h->prec == MFLBM_FP64)
+            Launch<double>::macro(sparse<double>(h), consts<double>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
+        else Launch<float>::macro(sparse<float>(h), consts<float>(h), h->d, dv[0], dv[1], dv[2], dv[3], h->stream);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(host[a], h->macro_buf, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
     return 0;
 }
 

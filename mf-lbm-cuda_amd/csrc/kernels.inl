@@ -732,7 +732,8 @@ __global__ __launch_bounds__(256) void k_aux_refresh(Fields<T> F, StepConst<T> P
 // Initial state (src/Init_multiphase.cpp:299-443)
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_init_phi(Fields<T> F, int option, double z0, double phi_inlet, int open_z) {
+__global__ __launch_bounds__(256) void k_init_phi(Fields<T> F, int option, double z0, double phi_inlet, int open_z,
+                                                  double sa_target, uint64_t seed) {
     const Dims& d = F.d;
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= d.S4) return;
@@ -742,6 +743,7 @@ __global__ __launch_bounds__(256) void k_init_phi(Fields<T> F, int option, doubl
         const double x = i, y = j, z = k;
         if (option == 1) v = (z <= z0) ? 1.0 : -1.0;
         else if (option == 2) v = (z <= z0) ? -1.0 : 1.0;
+        else if (option == 6) v = node_uniform(seed, (uint64_t)i1(d, i, j, k)) > sa_target ? -1.0 : 1.0;
         else {
             const double cx = (option == 5) ? (double)(d.nx + 1) * 0.5 : (double)(d.nx + 1) * 0.0;
             const double dx = x - cx, dz = z - (double)(d.nz + 1) * 0.5, dy = y - (double)(d.ny + 1) * 0.5;

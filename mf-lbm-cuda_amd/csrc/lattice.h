@@ -147,6 +147,18 @@ constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}, {
 constexpr int GRAD_NSHAPES = sizeof(GRAD_SHAPES) / sizeof(GRAD_SHAPES[0]);
 constexpr int GRAD_SHAPE_DEFAULT = 0, GRAD_SHAPE_FP32 = 3;
 
+// Uniform deviate in [0, 1) of lattice node n under a seed, for the random initial phase
+// distribution: the splitmix64 output function (Steele, Lea and Flood, OOPSLA 2014) of
+// seed + (n + 1) * 0x9E3779B97F4A7C15, top 53 bits.  Counter-based: every node draws its own
+// value, in any order, on the host or the device alike.
+MFLBM_HD double node_uniform(uint64_t seed, uint64_t n) {
+    uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
 constexpr int TAG_FLUID = 0, TAG_SOLID = 1, TAG_SOLID_BND = 2, TAG_WET = 3;
 MFLBM_HD bool tag_fluid(int code) { return code == TAG_FLUID || code == TAG_WET; }
 

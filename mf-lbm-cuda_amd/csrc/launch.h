@@ -24,7 +24,8 @@ struct Launch {
     static void aux_refresh(const Fields<T>& F, const StepConst<T>& P, const NodeLists& L, T* aux, hipStream_t s);
     static void cells(const Sparse<T>& S, T* dense, const Dims& d, const int32_t* ext_node, const int32_t* ext_off,
                       int n_ext, bool to_dense, hipStream_t s);
-    static void init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, hipStream_t s);
+    static void init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, double sa_target,
+                         uint64_t seed, hipStream_t s);
     static void init_eq(const Fields<T>& F, const double* rho, const double* u, const double* v, const double* w,
                         bool relaxed, double rho_ref, hipStream_t s);
     static void macro(const Sparse<T>& S, const StepConst<T>& P, const Dims& d, double* rho, double* u, double* v,

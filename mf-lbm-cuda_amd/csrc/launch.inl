@@ -123,8 +123,9 @@ void Launch<T>::cells(const Sparse<T>& S, T* dense, const Dims& d, const int32_t
 }
 
 template <typename T>
-void Launch<T>::init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, hipStream_t s) {
-    dev::k_init_phi<T><<<blocks(F.d.S4), 256, 0, s>>>(F, option, z0, phi_inlet, open_z);
+void Launch<T>::init_phi(const Fields<T>& F, int option, double z0, double phi_inlet, int open_z, double sa_target,
+                         uint64_t seed, hipStream_t s) {
+    dev::k_init_phi<T><<<blocks(F.d.S4), 256, 0, s>>>(F, option, z0, phi_inlet, open_z, sa_target, seed);
 }
 
 template <typename T>

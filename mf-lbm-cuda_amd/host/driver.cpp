@@ -78,6 +78,16 @@ public:
         }
         throw Fatal("ERROR! Couldn't find the variable named: (" + key + ") in the file " + path_);
     }
+    bool has(const std::string& key) const {
+        for (const std::string& line : lines_) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream ss(line);
+            std::string k;
+            ss >> k;
+            if (k == key) return true;
+        }
+        return false;
+    }
     long long i64(const std::string& key) const { return std::stoll(str(key)); }
     int i32(const std::string& key) const { return std::stoi(str(key)); }
     double real(const std::string& key) const { return std::stod(str(key)); }
@@ -96,7 +106,8 @@ struct Control {
     int max_step, ntime_visual, ntime_animation, ntime_monitor, monitor_profile_ratio;
     int ntime_clock_sum, ntime_display, rho_in_new, rho_out_bc, z_porous_plate;
     long long nx, ny, nz;
-    double convergence, nu1, nu2, gamma, theta, beta, sa_inject, interface_z0, body_force;
+    double convergence, nu1, nu2, gamma, theta, beta, sa_inject, interface_z0, body_force, sa_target;
+    uint64_t random_seed;
     double checkpoint_timer, checkpoint2_timer, duration_timer;
     double capillary_number, target_inject_pore_volume, d_vol_animation, d_vol_detail, d_vol_monitor;
     bool open_z() const { return kper == 0 && wall[4] == 0 && wall[5] == 0; }
@@ -145,6 +156,10 @@ Control read_control(const std::string& path) {
     c.outlet_bc = f.i32("outlet_BC");
     c.sa_inject = f.real("saturation_injection");
     c.interface_z0 = f.real("initial_interface_position");
+    c.sa_target = f.real("target_fluid1_saturation");
+    // option 6's generator seed: a key of this build (the reference seeds from the clock);
+    // absent, seed 1
+    c.random_seed = f.has("random_seed") ? std::stoull(f.str("random_seed")) : 1;
     c.body_force = f.real("body_force_0");
     c.max_step = f.i32("max_time_step");
     c.ntime_visual = f.i32("ntime_visual");
@@ -186,7 +201,9 @@ void check_control(const Control& c) {
     else if (c.geometry_preprocess != 0) err.push_back("Wrong value of geometry_preprocess_cmd");
     if (c.extreme_large != 0)
         err.push_back("Extreme large simulation domain (extreme_large_sim_cmd = 1) is not supported yet");
-    if (c.option < 1 || c.option > 5) err.push_back("initial_fluid_distribution_option must be 1-5");
+    if (c.option < 1 || c.option > 6) err.push_back("Input parameter initial_fluid_distribution_option error");
+    if (c.option == 6 && !(c.sa_target >= 0.0 && c.sa_target <= 1.0))
+        err.push_back("target_fluid1_saturation must lie in [0, 1]");
     if (c.ntime_monitor <= 0 || c.ntime_clock_sum <= 0 || c.ntime_display <= 0 || c.ntime_animation <= 0 ||
         c.ntime_visual <= 0 || c.monitor_profile_ratio <= 0)
         err.push_back("timers must be positive");
@@ -914,7 +931,8 @@ int Simulation::run(const std::string& job_status) {
     write_info();
     if (job_status == "new_simulation") {
         ntime0_ = 1;
-        check(mflbm_init_phi(h_, c_.option, c_.interface_z0), "init_phi");
+        if (c_.option == 6) check(mflbm_init_phi_random(h_, c_.sa_target, c_.random_seed), "init_phi_random");
+        else check(mflbm_init_phi(h_, c_.option, c_.interface_z0), "init_phi");
         check(mflbm_init_equilibrium(h_, nullptr, nullptr, nullptr, nullptr), "init_equilibrium");
     } else {
         upload_real(MFLBM_PDF, ck_pdf);

@@ -167,6 +167,8 @@ def derive(ctrl: dict) -> dict:
         "la_nu1": la_nu1, "la_nu2": la_nu2,
         "initial_fluid_distribution_option": int(ctrl["initial_fluid_distribution_option"]),
         "interface_z0": float(ctrl["initial_interface_position"]),
+        "sa_target": float(ctrl["target_fluid1_saturation"]),
+        "random_seed": int(ctrl.get("random_seed", 1)),
         "n_exclude_inlet": int(ctrl["n_exclude_inlet"]),
         "n_exclude_outlet": int(ctrl["n_exclude_outlet"]),
     }
@@ -203,3 +205,5 @@ def write_control(path: str, ctrl: dict) -> None:
             fh.write(f"{k} {int(ctrl[k])}\n")
         for k in REAL_KEYS:
             fh.write(f"{k} {float(ctrl[k])!r}\n")
+        if "random_seed" in ctrl:  # option 6's seed (a key of this build only)
+            fh.write(f"random_seed {int(ctrl['random_seed'])}\n")

@@ -23,7 +23,8 @@ HALO = {"pdf": 1, "phi": 4, "cn_x": 2, "cn_y": 2, "cn_z": 2, "c_norm": 2, "curv"
         "s_nx": 4, "s_ny": 4, "s_nz": 4, "walls": 2, "walls_type": 4}
 
 # Every symbol declared in include/mflbm.h.
-EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_equilibrium",
+EXPORTS = ["mflbm_create", "mflbm_destroy", "mflbm_last_error", "mflbm_init_phi", "mflbm_init_phi_random",
+           "mflbm_init_equilibrium",
            "mflbm_init_relaxed",
            "mflbm_color_gradient", "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload",
            "mflbm_macro", "mflbm_monitor", "mflbm_velocity_change", "mflbm_sync", "mflbm_count", "mflbm_field_size", "mflbm_bandwidth_probe",
@@ -80,6 +81,7 @@ def load():
     L.mflbm_last_error.argtypes = [vp]
     L.mflbm_last_error.restype = C.c_char_p
     L.mflbm_init_phi.argtypes = [vp, C.c_int32, C.c_double]
+    L.mflbm_init_phi_random.argtypes = [vp, C.c_double, C.c_uint64]
     L.mflbm_init_equilibrium.argtypes = [vp, dp, dp, dp, dp]
     L.mflbm_init_relaxed.argtypes = [vp, dp, dp, dp, dp]
     L.mflbm_color_gradient.argtypes = [vp]
@@ -112,7 +114,7 @@ def load():
     L.mflbm_ensemble_fields.restype = C.c_int
     L.mflbm_ensemble_init_fields.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
     L.mflbm_ensemble_init_fields.restype = C.c_int
-    for name in ("mflbm_init_phi", "mflbm_init_equilibrium", "mflbm_init_relaxed", "mflbm_color_gradient",
+    for name in ("mflbm_init_phi", "mflbm_init_phi_random", "mflbm_init_equilibrium", "mflbm_init_relaxed", "mflbm_color_gradient",
                  "mflbm_step", "mflbm_run", "mflbm_profile", "mflbm_download", "mflbm_upload", "mflbm_macro",
                  "mflbm_monitor", "mflbm_sync"):
         getattr(L, name).restype = C.c_int
@@ -182,6 +184,10 @@ class Solver:
     def init_phi(self, option, interface_z0):
         self._check(load().mflbm_init_phi(self.h, int(option), float(interface_z0)))
 
+    def init_phi_random(self, sa_target, seed):
+        """option 6: phi = -1 where the node's seeded deviate exceeds sa_target"""
+        self._check(load().mflbm_init_phi_random(self.h, float(sa_target), int(seed)))
+
     def init_equilibrium(self, rho=None, u=None, v=None, w=None, relaxed=False):
         """relaxed: rest weights of the collision (mflbm_init_relaxed) instead of the D3Q19 ones"""
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w)]
@@ -192,8 +198,11 @@ class Solver:
         self._check(load().mflbm_color_gradient(self.h))
 
     def setup(self, option=None, interface_z0=None):
-        self.init_phi(self.p["initial_fluid_distribution_option"] if option is None else option,
-                      self.p["interface_z0"] if interface_z0 is None else interface_z0)
+        option = self.p["initial_fluid_distribution_option"] if option is None else option
+        if option == 6:
+            self.init_phi_random(self.p["sa_target"], self.p["random_seed"])
+        else:
+            self.init_phi(option, self.p["interface_z0"] if interface_z0 is None else interface_z0)
         self.init_equilibrium()
         self.color_gradient()
 

@@ -345,7 +345,18 @@ void preprocess_geometry(OracleSim& S, const int8_t* geo) {
 // =====================================================================================
 //  Initial state (src/Init_multiphase.cpp:299-443)
 // =====================================================================================
-void init_phi(OracleSim& S, int option, double interface_z0) {
+// Option 6's deviate of node n: splitmix64 output function (Steele, Lea and Flood 2014) of
+// seed + (n + 1) * 0x9E3779B97F4A7C15, top 53 bits, in [0, 1).  The reference draws
+// rand() / RAND_MAX seeded from time(NULL) (src/Init_multiphase.cpp:306, 344-350).
+double node_uniform(uint64_t seed, uint64_t n) {
+    uint64_t z = seed + (n + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
+void init_phi(OracleSim& S, int option, double interface_z0, double sa_target = 0.0, uint64_t seed = 0) {
     const int64_t nx = S.nx, ny = S.ny, nz = S.nz;
     std::fill(S.phi.begin(), S.phi.end(), 0.0);
     for (int64_t k = 0; k <= nz + 1; ++k)
@@ -363,6 +374,9 @@ void init_phi(OracleSim& S, int option, double interface_z0) {
                     bool inside = dx * dx + dz * dz + dy * dy <= interface_z0 * interface_z0;
                     if (option == 4) v = inside ? -1.0 : 1.0;
                     else v = inside ? 1.0 : -1.0;
+                } else if (option == 6) {
+                    const uint64_t n = (uint64_t)(i + (nx + 2) * (j + (ny + 2) * k));  // s1 index
+                    v = node_uniform(seed, n) > sa_target ? -1.0 : 1.0;
                 }
                 S.phi[S.s4(i, j, k)] = v;
             }
@@ -1123,8 +1137,15 @@ void* oracle_create(const oracle_params* p, const int8_t* geometry) {
 
 void oracle_destroy(void* h) { delete static_cast<OracleSim*>(h); }
 
+int oracle_init_phi_random(void* h, double sa_target, uint64_t seed) {
+    init_phi(*static_cast<OracleSim*>(h), 6, 0.0, sa_target, seed);
+    return 0;
+}
+
+double oracle_node_uniform(uint64_t seed, uint64_t n) { return node_uniform(seed, n); }
+
 int oracle_init_phi(void* h, int option, double interface_z0) {
-    if (option < 1 || option > 5) return -1;  // option 6 is seeded from the wall clock in the reference
+    if (option < 1 || option > 5) return -1;  // option 6: oracle_init_phi_random
     init_phi(*static_cast<OracleSim*>(h), option, interface_z0);
     return 0;
 }

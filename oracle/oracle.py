@@ -54,6 +54,10 @@ def lib():
         L.oracle_destroy.argtypes = [vp]
         L.oracle_init_phi.argtypes = [vp, C.c_int, C.c_double]
         L.oracle_init_phi.restype = C.c_int
+        L.oracle_init_phi_random.argtypes = [vp, C.c_double, C.c_uint64]
+        L.oracle_init_phi_random.restype = C.c_int
+        L.oracle_node_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.oracle_node_uniform.restype = C.c_double
         L.oracle_init_pdf.argtypes = [vp, dp, dp, dp, dp]
         L.oracle_init_pdf_relaxed.argtypes = [vp, dp, dp, dp, dp]
         L.oracle_color_gradient.argtypes = [vp]
@@ -151,6 +155,10 @@ class Oracle:
         if lib().oracle_init_phi(self.h, int(option), float(interface_z0)) != 0:
             raise ValueError(f"unsupported initial_fluid_distribution_option {option}")
 
+    def init_phi_random(self, sa_target, seed):
+        """option 6: phi = -1 where the node's seeded deviate exceeds sa_target"""
+        lib().oracle_init_phi_random(self.h, float(sa_target), int(seed))
+
     def init_pdf(self, rho=None, u=None, v=None, w=None, relaxed=False):
         """relaxed: rest weights 4/21, 13/126, 1/63 (the collision's own rest state)"""
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w)]
@@ -193,8 +201,11 @@ class Oracle:
 
     def setup(self, option=None, interface_z0=None):
         """New-simulation initialisation: phi, equilibrium populations, initial gradient."""
-        self.init_phi(self.p["initial_fluid_distribution_option"] if option is None else option,
-                      self.p["interface_z0"] if interface_z0 is None else interface_z0)
+        option = self.p["initial_fluid_distribution_option"] if option is None else option
+        if option == 6:
+            self.init_phi_random(self.p["sa_target"], self.p["random_seed"])
+        else:
+            self.init_phi(option, self.p["interface_z0"] if interface_z0 is None else interface_z0)
         self.init_pdf()
         self.color_gradient()
 

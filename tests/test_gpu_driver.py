@@ -256,3 +256,71 @@ def test_driver_modified_geometry_and_inlet_phase_switch(tmp_path):
     nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
     assert nt == 101 and np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
     ref.close()
+
+
+def test_driver_breakthrough_ends_the_run(tmp_path):
+    """breakthrough_check 1: the run stops at the first monitor step at which fluid 1
+    (phi > 0) holds a fluid node of the plane nz - 1 (src/Monitor.cpp:447-468), saves the
+    state and reports simulation_done (src/main.cpp:296-312).  The Python API run of the
+    same case has no fluid 1 on that plane one monitor interval earlier."""
+    root = str(tmp_path)
+    n = (16, 16, 32)
+    nx, ny, nz = n
+    ctrl = cases.drainage_control(n=n, interface_z0=float(nz - 8), body_force=5e-4)
+    ctrl.update(breakthrough_check=1, steady_state_option=0, max_time_step=6000, monitor_timer=20,
+                monitor_profile_timer_ratio=1000, ntime_visual=100000, ntime_animation=100000,
+                computation_time_timer=1000, display_steps_timer=1000, n_exclude_inlet=3, n_exclude_outlet=3,
+                checkpoint_save_timer=1e3, checkpoint_2rd_save_timer=1e3, simulation_duration_timer=1e3)
+    geo = cases.sphere_geometry(n, 3.0, 0.8, 5, buffer=3)
+    write_case(root, ctrl, geo)
+    say("driver: drainage to breakthrough")
+    out = run_driver(root)
+    assert "Breakthrough point reached! Exiting program!" in out
+    t = int(out.split("Simulation ended successfully after\t")[1].split()[0])
+    say(f"breakthrough at step {t}")
+    assert t % 20 == 0 and 20 < t < 6000
+    assert open(os.path.join(root, "job_status.txt")).read() == "simulation_done\n"
+    assert os.path.exists(os.path.join(root, f"results/out3.field_data/phase_distribution/small_{t:010d}.vtk"))
+
+    ref = reference_run(ctrl, geo, t - 20)
+    walls = ref.download("walls")[nz, 2:ny + 2, 2:nx + 2] == 0  # plane nz - 1 (two ghost layers)
+
+    def fluid1_on_plane():
+        return int((walls & (ref.download("phi")[nz + 2, 4:ny + 4, 4:nx + 4] > 0.0)).sum())
+
+    assert fluid1_on_plane() == 0
+    ref.run(t - 19, 20)
+    assert fluid1_on_plane() >= 1
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == t + 1
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    ref.close()
+
+
+def test_driver_random_initial_distribution(tmp_path):
+    """initial_fluid_distribution_option 6 from the control file with this build's
+    random_seed key: the run equals the Python API run from the same seeded field, and the
+    initial saturation is the target's."""
+    root = str(tmp_path)
+    n = (16, 16, 20)
+    nx, ny, nz = n
+    ctrl = cases.periodic_control(n=n, body_force=1e-5, option=6, interface_z0=0.0)
+    ctrl.update(target_fluid1_saturation=0.3, random_seed=77, max_time_step=99, monitor_timer=50,
+                monitor_profile_timer_ratio=100, ntime_visual=100000, ntime_animation=100000,
+                computation_time_timer=50, display_steps_timer=100, n_exclude_inlet=0, n_exclude_outlet=0,
+                steady_state_option=0, checkpoint_save_timer=1e3, checkpoint_2rd_save_timer=1e3,
+                simulation_duration_timer=1e3)
+    geo = np.zeros((nz, ny, nx), np.int8)
+    write_case(root, ctrl, geo)
+    say("driver: option 6, seed 77")
+    out = run_driver(root)
+    s0 = float(out.split("Initial saturation: ")[1].split()[0])
+    assert abs(s0 - 0.3) < 0.03, s0
+    ref = reference_run(ctrl, geo, 100)
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 101
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    assert np.array_equal(body[2 + 38 * s1:], ref.download("phi").ravel())
+    ref.close()

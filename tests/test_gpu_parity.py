@@ -155,6 +155,31 @@ def test_relaxed_init_bitwise():
     compare_state(o, g, tag="relaxed step20")
 
 
+@pytest.mark.parametrize("open_z", [True, False])
+def test_random_initial_distribution_bitwise(open_z):
+    """Option 6 (mflbm_init_phi_random): the seeded random phase field, the populations
+    built on it and 30 steps of its break-up agree bit for bit with the oracle."""
+    n = (20, 18, 24)
+    nx, ny, nz = n
+    if open_z:
+        ctrl = cases.drainage_control(n=n, interface_z0=3.0)
+        geo = cases.sphere_geometry(n, 3.0, 0.7, 4, buffer=3)
+    else:
+        ctrl = cases.periodic_control(n=n, body_force=1e-5, option=6)
+        geo = np.zeros((nz, ny, nx), np.int8)
+    ctrl.update(initial_fluid_distribution_option=6, target_fluid1_saturation=0.35, random_seed=2024)
+    p = config.derive(ctrl)
+    say(f"case=option 6 open_z={open_z}")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    assert_same(g.download("phi"), o.array("phi"), what="initial phi")
+    compare_state(o, g, tag="setup")
+    o.run(1, 30)
+    g.run(1, 30)
+    compare_state(o, g, tag="step 30")
+
+
 def test_fused_gradient_short_z_runs(monkeypatch):
     """Workgroups marching 5-plane z-runs (many run boundaries) still match the oracle."""
     monkeypatch.setenv("MFLBM_GRAD_KCHUNK", "5")
