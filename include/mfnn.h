@@ -48,6 +48,19 @@ int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_
 int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
                      int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream);
 
+/* mfnn_conv3_wgrad without the workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same kernels write their fp32 partials
+ * to a fixed array of MFNN_WGRAD_SCRATCH_FLOATS floats (64 MiB) that the library holds on
+ * every device it runs on until the process ends; the call allocates nothing.  Returns -3
+ * (and launches nothing) when mfnn_wgrad_workspace(D, H, W, kp, cout, chunk) exceeds that
+ * size, otherwise as mfnn_conv3_wgrad, with results bit-identical to it.  Calls on one
+ * device share its array: calls that overlap on one device (other streams, other host
+ * threads) race and must be ordered by the caller. */
+#define MFNN_WGRAD_SCRATCH_FLOATS (16 * 1024 * 1024)
+int mfnn_conv3_wgrad_static(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
+                            int32_t cout, int32_t chunk, float* dw, float* db, int32_t accumulate, void* stream);
+
 /* wt[ci][tap' * cout + co] = w[co][(26 - tap') * cin + ci] (rows padded to cinp, a multiple of
  * 16; row length kp2, a multiple of 32 >= 27 * cout): the weights of the data gradient. */
 int mfnn_flip_weights(const void* w, int32_t cin, int32_t kp, int32_t cout, int32_t kp2, int32_t cinp, void* wt,

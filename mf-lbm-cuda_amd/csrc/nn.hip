@@ -17,6 +17,7 @@
 // copy plus a bf16 copy for the MFMA operands.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/mfnn.h"
 
@@ -428,11 +429,20 @@ int64_t mfnn_wgrad_workspace(int32_t D, int32_t H, int32_t W, int32_t kp, int32_
     return nchunk * ((int64_t)((cout + 15) / 16 * 16) * kp + cout);  // weight partials, then bias partials
 }
 
-int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
-                     int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream) {
-    if (cin % 8 || cout % 8 || cin <= 0 || cout <= 0 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return -1;
-    if (D <= 0 || H <= 0 || W <= 0) return -1;
-    if ((uintptr_t)x % 16 || (uintptr_t)dz % 16) return -1;  // 16-byte operand loads
+}  // extern "C"
+
+namespace {
+
+bool wgrad_args_ok(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
+                   int32_t cout, int32_t chunk) {
+    if (cin % 8 || cout % 8 || cin <= 0 || cout <= 0 || kp % 32 || kp < 27 * cin || chunk <= 0 || chunk % 32) return false;
+    if (D <= 0 || H <= 0 || W <= 0) return false;
+    return (uintptr_t)x % 16 == 0 && (uintptr_t)dz % 16 == 0;  // 16-byte operand loads
+}
+
+// The weight-gradient launches of valid arguments: partials into `work`, then the chunk sums.
+int wgrad_launch(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp, int32_t cout,
+                 int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream) {
     const Grid3 g{D, H, W};
     const int64_t M = (int64_t)D * H * W;
     const int nchunk = (int)((M + chunk - 1) / chunk);
@@ -451,6 +461,47 @@ int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_
     const unsigned sblocks = nblk(n, 32) + (db ? nblk(cout, 32) : 0);
     k_sum_chunks<<<sblocks, 256, 0, s>>>(work, nchunk, n, bpart, db ? cout : 0, accumulate, dw, db);
     return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// Scratch of mfnn_conv3_wgrad_static: every partial it sums is written first by the
+// partial kernels of the same call, so no call needs it cleared.  In host code `scratch`
+// names a host-side shadow, not the device array: the device address comes from
+// hipGetSymbolAddress, once per device (each device holds its own copy of the array).
+static __device__ __align__(256) float scratch[MFNN_WGRAD_SCRATCH_FLOATS];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+float* scratch_dev[MAX_DEVICES] = {};
+
+// Address of the current device's scratch (null on a runtime error).
+float* scratch_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        scratch_dev[dev] = static_cast<float*>(p);
+    }
+    return scratch_dev[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfnn_conv3_wgrad(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
+                     int32_t cout, int32_t chunk, float* work, float* dw, float* db, int32_t accumulate, void* stream) {
+    if (!wgrad_args_ok(x, dz, D, H, W, cin, kp, cout, chunk)) return -1;
+    return wgrad_launch(x, dz, D, H, W, cin, kp, cout, chunk, work, dw, db, accumulate, stream);
+}
+
+int mfnn_conv3_wgrad_static(const void* x, const void* dz, int32_t D, int32_t H, int32_t W, int32_t cin, int32_t kp,
+                            int32_t cout, int32_t chunk, float* dw, float* db, int32_t accumulate, void* stream) {
+    if (!wgrad_args_ok(x, dz, D, H, W, cin, kp, cout, chunk)) return -1;
+    if (mfnn_wgrad_workspace(D, H, W, kp, cout, chunk) > (int64_t)MFNN_WGRAD_SCRATCH_FLOATS) return -3;
+    float* work = scratch_address();
+    if (!work) return -2;
+    return wgrad_launch(x, dz, D, H, W, cin, kp, cout, chunk, work, dw, db, accumulate, stream);
 }
 
 int mfnn_flip_weights(const void* w, int32_t cin, int32_t kp, int32_t cout, int32_t kp2, int32_t cinp, void* wt,

@@ -38,6 +38,7 @@ SIGNATURES = {
     "mfnn_conv3": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp],
     "mfnn_wgrad_workspace": [_i32, _i32, _i32, _i32, _i32, _i32],
     "mfnn_conv3_wgrad": [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp],
+    "mfnn_conv3_wgrad_static": [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp],
     "mfnn_flip_weights": [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp],
     "mfnn_to_bf16": [_vp, _i64, _vp, _vp],
     "mfnn_pool2": [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp],

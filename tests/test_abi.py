@@ -57,6 +57,13 @@ def test_network_argument_checks_reject_before_any_launch():
     # weight gradient: an empty grid
     assert L.mfnn_conv3_wgrad(aligned, aligned, 0, 4, 4, 8, 224, 8, 32, None, None, None, 0, None) == -1
     assert L.mfnn_conv3_wgrad(aligned, aligned, 4, 4, -1, 8, 224, 8, 32, None, None, None, 0, None) == -1
+    # the weight gradient without a workspace: the same checks, and -3 for a workspace
+    # larger than its fixed scratch (16 Mi floats), all before touching the device
+    assert L.mfnn_conv3_wgrad_static(aligned, aligned, 4, 4, 4, 8, 224, 12, 32, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad_static(aligned + 2, aligned, 4, 4, 4, 8, 224, 8, 32, None, None, 0, None) == -1
+    assert L.mfnn_conv3_wgrad_static(aligned, aligned, 0, 4, 4, 8, 224, 8, 32, None, None, 0, None) == -1
+    assert L.mfnn_wgrad_workspace(64, 64, 64, 1728, 128, 2048) > 16 * 1024 * 1024
+    assert L.mfnn_conv3_wgrad_static(aligned, aligned, 64, 64, 64, 64, 1728, 128, 2048, None, None, 0, None) == -3
     # forward: more than 192 output channels, an empty grid, misaligned input or weights
     assert L.mfnn_conv3(aligned, aligned, None, None, 4, 4, 4, 8, 224, 200, 1, aligned, None, None) == -1
     assert L.mfnn_conv3(aligned, aligned, None, None, 0, 4, 4, 8, 224, 16, 1, aligned, None, None) == -1
