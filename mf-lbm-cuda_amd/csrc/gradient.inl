@@ -28,6 +28,7 @@
 // wall columns, and the periodic copies).  The full dense gradient
 // fields are produced on request by the split kernels (they are functions of phi alone).
 #pragma once
+#include <type_traits>
 #include "kernels.inl"
 
 namespace mflbm {
@@ -95,11 +96,13 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
         in = j < SEG && e < NP;
         return e;
     };
-    // register staging of the next plane: its loads are in flight while the stages run
-    // (kept raw until the store, so that nothing waits for them early)
-    T pv[NSUB];
-    int32_t tv[NSUB];
-    auto fetch = [&](int p) {
+    // register staging of the planes two ahead of the store: two register sets, the loop
+    // unrolled by two, so that a plane's loads have two iterations of stage work to arrive
+    // in (kept raw until the store, so that nothing waits for them early).  Against one
+    // plane ahead: 0.680 -> 0.652 ms at 256^3 FP64, 0.546 -> 0.496 ms in FP32.
+    T pvb[2][NSUB];
+    int32_t tvb[2][NSUB];
+    auto fetch = [&](int p, T (&pv)[NSUB], int32_t (&tv)[NSUB]) {
 #pragma unroll
         for (int sb = 0; sb < NSUB; ++sb) {
             bool in;
@@ -137,11 +140,15 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     if (tid < 2) s_cnt[tid] = 0;
     __syncthreads();
 
-    fetch(k0 - H);
+    fetch(k0 - H, pvb[0], tvb[0]);
+    if (k0 - H + 1 <= k1 + H) fetch(k0 - H + 1, pvb[1], tvb[1]);
     int32_t tcur[NSUB];
 #pragma unroll
     for (int sb = 0; sb < NSUB; ++sb) tcur[sb] = TAG_SOLID;
-    for (int p = k0 - H; p <= k1 + H; ++p) {
+    auto body = [&](int p, auto B) {
+        constexpr int b = decltype(B)::value;
+        T (&pv)[NSUB] = pvb[b];
+        int32_t (&tv)[NSUB] = tvb[b];
         const int par = p & 1;
 #pragma unroll
         for (int sb = 0; sb < NSUB; ++sb) {
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 if (in) ph[e] = pv[sb];
             }
         }
-        if (p < k1 + H) fetch(p + 1);
+        if (p + 2 <= k1 + H) fetch(p + 2, pv, tv);
         // the normals slot K4 fills in this iteration held plane p-6, which no stage reads
         // any more; its flag starts over
         if (tid == 0) s_any[s4(p - 2)] = 0;
@@ -363,6 +370,10 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 a[3] = T(0.5) * P.gamma * curv * c_norm;
             }
         }
+    };
+    for (int p = k0 - H; p <= k1 + H; p += 2) {
+        body(p, std::integral_constant<int, 0>{});
+        if (p + 1 <= k1 + H) body(p + 1, std::integral_constant<int, 1>{});
     }
 }
 

@@ -256,38 +256,27 @@ __global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, 
                                                                    StepConst<T> P) {
     const int t = xcd_block() * MFLBM_COLLIDE_THREADS + threadIdx.x;
     if (t >= S.nf) return;
-    // The node's own words first, so that their loads are in flight together with the
-    // neighbour row (odd) or the populations (even) instead of after them: the force
-    // record, which the collision needs early, and the packed coordinates of the phi
-    // store.  The neighbour row, force record and phi of the odd step are touched once;
-    // nontemporal accesses leave the cache to the shared lines of the scattered cells
-    // (6 % faster; in the even kernel they cost 2-5 %).
+    // The neighbour row, force record and phi of the odd step are touched once; nontemporal
+    // accesses leave the cache to the shared lines of the scattered cells (6 % faster; in
+    // the even kernel they cost 2-5 %).  The odd step loads the node's own words (force
+    // record, packed coordinates) with its neighbour row, held there by a scheduling
+    // barrier: sunk to their first use they were issued after the population loads had
+    // returned, a third round trip to memory (0.6 % on the odd launch; the same order in
+    // the even kernel measured 0.8 % slower and is not used there).
     const T* a = S.aux + 4 * (int64_t)t;
     T rec[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) rec[m] = ODD ? __builtin_nontemporal_load(a + m) : a[m];
-    const uint32_t packed = S.list[t];
-    // keep the scheduler from sinking those loads to their first use (behind the
-    // population loads, a second round trip to memory)
-    __builtin_amdgcn_sched_barrier(0);
+    uint32_t packed = 0;
     T g1[19], g2[19], o1[19], o2[19];
     int nb[19];
     if (ODD) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) rec[m] = __builtin_nontemporal_load(a + m);
+        packed = S.list[t];
         nb[0] = t;
 #pragma unroll
         for (int q = 1; q < 19; ++q) nb[q] = __builtin_nontemporal_load(S.nb + (int64_t)(q - 1) * S.nf + t);
+        __builtin_amdgcn_sched_barrier(0);
     }
-#ifndef MFLBM_ODD_NB_IN_REGISTERS
-    // the odd step parks the neighbour row in LDS while it computes and reads it back for
-    // the pushes: 18 fewer live registers (139 instead of 190), 3 waves per SIMD instead
-    // of 2.  Each thread reads back only its own words (volatile: not forwarded from the
-    // stores, which would keep the registers live).
-    __shared__ int s_nb[ODD ? 18 : 1][MFLBM_COLLIDE_THREADS];
-    if (ODD) {
-#pragma unroll
-        for (int q = 1; q < 19; ++q) s_nb[q - 1][threadIdx.x] = nb[q];
-    }
-#endif
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
         const int64_t src = ODD ? (int64_t)nb[OPP[q]] : (int64_t)t;
@@ -303,17 +292,16 @@ __global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, 
             g2[q] = __builtin_nontemporal_load(S.f + S.at(slot + 19, src));
         }
     }
+    if (!ODD) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) rec[m] = a[m];
+        packed = S.list[t];
+    }
     const T ph = collide_node<T>(P, g1, g2, rec[0], rec[1], rec[2], rec[3], o1, o2);
     int i, j, k;
     unpack_ijk(packed, i, j, k);
     if (ODD) __builtin_nontemporal_store(ph, phi + i4(d, i, j, k));
     else phi[i4(d, i, j, k)] = ph;
-#ifndef MFLBM_ODD_NB_IN_REGISTERS
-    if (ODD) {
-#pragma unroll
-        for (int q = 1; q < 19; ++q) nb[q] = ((volatile int*)s_nb[q - 1])[threadIdx.x];
-    }
-#endif
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
         const int64_t dst = ODD ? (int64_t)nb[q] : (int64_t)t;
