@@ -23,6 +23,7 @@
 #include "geometry.h"
 #include "lattice.h"
 #include "launch.h"
+#include "device_lock.h"
 #include "layout.h"
 
 using namespace mflbm;
@@ -272,6 +273,7 @@ int field_info(const mflbm_handle* h, int field, int64_t* count, size_t* elem, v
 }
 
 int build_graphs(mflbm_handle* h) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     for (int g = 0; g < 2; ++g) {
         if (h->graph[g]) continue;
         hipGraph_t graph = nullptr;
@@ -287,6 +289,7 @@ int build_graphs(mflbm_handle* h) {
 }
 
 void invalidate_graphs(mflbm_handle* h) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     for (int g = 0; g < 2; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
 }
@@ -347,6 +350,7 @@ extern "C" {
 const char* mflbm_last_error(const mflbm_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
 int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handle** out) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!out) return -1;
     *out = nullptr;
     mflbm_handle* h = new mflbm_handle();
@@ -570,6 +574,7 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
 }
 
 void mflbm_destroy(mflbm_handle* h) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!h) return;
     free_all(h);
     delete h;
@@ -601,6 +606,7 @@ int mflbm_init_phi_random(mflbm_handle* h, double sa_target, uint64_t seed) {
 
 static int init_populations(mflbm_handle* h, const double* rho, const double* u, const double* v, const double* w,
                             bool relaxed) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!h) return -1;
     h->vprev_valid = false;
     const int64_t n = h->d.S1;
@@ -766,6 +772,7 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
 }
 
 int mflbm_macro(mflbm_handle* h, double* rho, double* u, double* v, double* w) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!h) return -1;
     const int64_t n = h->d.S1;
     double* host[4] = {rho, u, v, w};
@@ -815,6 +822,7 @@ int mflbm_monitor(mflbm_handle* h, mflbm_monitor_out* out) {
 }
 
 int mflbm_velocity_change(mflbm_handle* h, double* change) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!h || !change) return -1;
     const int64_t nb = (h->n_fluid + 255) / 256;
     if (!h->vprev || !h->vpart) {  // first call (or a first call whose allocation failed)
@@ -848,6 +856,7 @@ int64_t mflbm_count(const mflbm_handle* h, int32_t which) {
 }
 
 int mflbm_bandwidth_probe(mflbm_handle* h, int64_t bytes, int32_t repeats, double* ms_per_copy) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!h || !ms_per_copy || bytes <= 0 || repeats <= 0) return -1;
     const int64_t n = bytes / (int64_t)sizeof(double4);
     double4 *a = nullptr, *b = nullptr;

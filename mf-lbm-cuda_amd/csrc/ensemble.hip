@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/mflbm.h"
+#include "device_lock.h"
 #include "kernels.inl"
 
 using namespace mflbm;
@@ -283,6 +284,7 @@ extern "C" {
 
 int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, int32_t n_domains, double* rho_ref,
                           mflbm_ensemble** out) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!out) return -1;
     *out = nullptr;
     auto* e = new mflbm_ensemble();
@@ -413,6 +415,7 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
 }
 
 void mflbm_ensemble_destroy(mflbm_ensemble* e) {
+    std::lock_guard<std::recursive_mutex> lock(device_lock());
     if (!e) return;
     free_all(e);
     delete e;

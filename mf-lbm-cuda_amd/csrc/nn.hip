@@ -20,6 +20,7 @@
 #include <mutex>
 
 #include "../../include/mfnn.h"
+#include "device_lock.h"
 
 namespace {
 
@@ -361,12 +362,16 @@ inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b
 extern "C" {
 
 void* mfnn_alloc(int64_t bytes) {
+    std::lock_guard<std::recursive_mutex> lock(mflbm::device_lock());
     void* p = nullptr;
     if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
     return p;
 }
 
-int mfnn_free(void* p) { return hipFree(p) == hipSuccess ? 0 : -2; }
+int mfnn_free(void* p) {
+    std::lock_guard<std::recursive_mutex> lock(mflbm::device_lock());
+    return hipFree(p) == hipSuccess ? 0 : -2;
+}
 
 int mfnn_copy(void* dst, const void* src, int64_t bytes, int32_t kind) {
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
