@@ -7,7 +7,7 @@ quantities the reference derives (src/Monitor.cpp:17-273), all reduced in FP64 o
 * saturation S = sum(vol1) / (sum(vol1) + sum(vol2));
 * fluid fluxes fl1, fl2 (w-flux summed over slices, averaged over nz) and the capillary
   number Ca = (fl1_avg / A_xy) * nu1 / gamma (src/Monitor.cpp:170-171);
-* total mass M = sum(mass1 + mass2).
+* total mass M = sum(mass1 + mass2), and each colour's mass on its own.
 
 The phase fluxes under the same driving force stand for the relative permeabilities
 (k_i = nu_i u_i / grad p), so their relative errors are the relative-permeability errors.
@@ -35,7 +35,8 @@ def summary(mon: dict, params: dict) -> dict:
     nu1 = 1.0 / params["la_nui1"]
     return {"S": vol1 / max(vol1 + vol2, 1e-300), "fl1": fl1, "fl2": fl2,
             "Ca": fl1 / axy * nu1 / params["lbm_gamma"],
-            "mass": float(np.sum(mon["mass1"]) + np.sum(mon["mass2"]))}
+            "mass": float(np.sum(mon["mass1"]) + np.sum(mon["mass2"])),
+            "mass1": float(np.sum(mon["mass1"])), "mass2": float(np.sum(mon["mass2"]))}
 
 
 def rel(a, b):
@@ -70,10 +71,12 @@ def run(case="grf_drainage512", steps=50000, interval=1000, n=None, log=print) -
                "Ca64": a["Ca"], "Ca32": b["Ca"], "Ca_rel": rel(b["Ca"], a["Ca"]),
                "mass_drift64": (a["mass"] - m0["fp64"]) / m0["fp64"],
                "mass_drift32": (b["mass"] - m0["fp32"]) / m0["fp32"],
-               "mass_rel": rel(b["mass"], a["mass"])}
+               "mass_rel": rel(b["mass"], a["mass"]),
+               "mass1_rel": rel(b["mass1"], a["mass1"]), "mass2_rel": rel(b["mass2"], a["mass2"])}
         series.append(row)
         log(f"step {row['step']}: S64 {row['S64']:.6f} S32 {row['S32']:.6f} dS {row['dS']:+.2e} "
-            f"fl1 {row['fl1_rel']:.2e} fl2 {row['fl2_rel']:.2e} mass {row['mass_rel']:.2e}")
+            f"fl1 {row['fl1_rel']:.2e} fl2 {row['fl2_rel']:.2e} mass {row['mass_rel']:.2e} "
+            f"mass1 {row['mass1_rel']:.2e} mass2 {row['mass2_rel']:.2e}")
     for s in sims.values():
         s.close()
     nodes = params["nx"] * params["ny"] * params["nz"]
@@ -82,9 +85,11 @@ def run(case="grf_drainage512", steps=50000, interval=1000, n=None, log=print) -
         "case": case, "lattice": [params["nx"], params["ny"], params["nz"]], "steps": steps,
         "interval": interval, "reference": "FP64 build on the device (bit-identical to the FP64 oracle)",
         "final": {k: last[k] for k in ("S64", "S32", "dS", "fl1_rel", "fl2_rel", "Ca64", "Ca32", "Ca_rel",
-                                       "mass_drift64", "mass_drift32", "mass_rel")},
+                                       "mass_drift64", "mass_drift32", "mass_rel", "mass1_rel", "mass2_rel")},
         "max_over_run": {"abs_dS": max(abs(r["dS"]) for r in series),
-                         "mass_rel": max(r["mass_rel"] for r in series)},
+                         "mass_rel": max(r["mass_rel"] for r in series),
+                         "mass1_rel": max(r["mass1_rel"] for r in series),
+                         "mass2_rel": max(r["mass2_rel"] for r in series)},
         "mlups": {k: nodes * steps / v / 1e6 for k, v in t_run.items()},
         "series": series,
     }
