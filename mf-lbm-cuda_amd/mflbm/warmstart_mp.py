@@ -34,7 +34,6 @@ import argparse
 import json
 import sys
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -154,25 +153,20 @@ def steady_field(geo, params, precision=gpu.FP32, **kw):
     return steps, s, phi
 
 
-def generate(seeds, n=64, workers=4, log=print, **kw):
+def generate(seeds, n=64, log=print, **kw):
     """[(geometry, steady phi, params, meta)]: the reference's start of each seed run to its
-    steady saturation (FP32), `workers` solves at a time on their own streams."""
-    def one(seed):
+    steady saturation (FP32), one solve after the other on one host thread."""
+    data = []
+    for seed in seeds:
         geo, params, meta = drainage_case(seed, n)
         t0 = time.perf_counter()
-        steps, s, phi = steady_field(geo, params, **kw)
+        steps, s, phi = steady_field(geo, params, log=lambda m: log(f"sample {seed}:{m}"), **kw)
         meta.update(steps=steps, saturation=s, seconds=time.perf_counter() - t0)
-        return geo, phi, params, meta
-
-    data = []
-    with ThreadPoolExecutor(max_workers=workers) as pool:
-        for geo, phi, params, meta in pool.map(one, list(seeds)):
-            log(f"sample {meta['seed']}: steady S {meta['saturation']:.4f} after {meta['steps']} steps "
-                f"({meta['seconds']:.1f} s)")
-            if np.isfinite(meta["saturation"]) and np.isfinite(phi).all():
-                data.append((geo, phi, params, meta))
-            else:
-                log(f"sample {meta['seed']} dropped: the run blew up")
+        log(f"sample {seed}: steady S {s:.4f} after {steps} steps ({meta['seconds']:.1f} s)")
+        if np.isfinite(s) and np.isfinite(phi).all():
+            data.append((geo, phi, params, meta))
+        else:
+            log(f"sample {seed} dropped: the run blew up")
     return data
 
 
@@ -355,7 +349,6 @@ def main(argv=None):
     ap.add_argument("--fine", action="store_true", help="evaluate on geometries twice as fine as --n")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--check", type=int, default=2000)
-    ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--save", default=None)
     ap.add_argument("--load", default=None)
     ap.add_argument("--out", default=None)
@@ -366,7 +359,7 @@ def main(argv=None):
         ws = PhaseWarmStart.load(args.load)
     else:
         t0 = time.perf_counter()
-        data = generate(range(args.train), args.n, args.workers, log, tol=args.tol, check=args.check)
+        data = generate(range(args.train), args.n, log, tol=args.tol, check=args.check)
         res["data_seconds"] = time.perf_counter() - t0
         res["train_steady_steps_mean"] = float(np.mean([m["steps"] for _, _, _, m in data]))
         ws = PhaseWarmStart()
