@@ -138,6 +138,7 @@ GradArgs<T> grad_args(const mflbm_handle* h) {
     A.tag = h->tag;
     A.snx = (const T*)h->snx; A.sny = (const T*)h->sny; A.snz = (const T*)h->snz;
     A.aux = (T*)h->aux;
+    A.cnx = (T*)h->cnx; A.cny = (T*)h->cny; A.cnz = (T*)h->cnz; A.cnorm = (T*)h->cnorm;
     A.shape = h->grad_shape; A.tiles_x = h->tiles_x; A.tiles_y = h->tiles_y; A.kchunk = h->kchunk;
     return A;
 }
@@ -502,20 +503,28 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
                 }
         HIPCHK(h, upload_i32_raw(&h->tag, tag));
         h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
-        // FP32 runs 6 waves per workgroup (3 per SIMD): 5 % faster there; FP64 keeps 4 waves
-        // (6 measured 60 % slower)
+        // Gradient chain.  The z-march (one pass, LDS rings) advances one plane per ~5 us
+        // whatever the tile area, so it pays where its z-runs are long; the two-pass 3-D
+        // tiles launch thousands of small workgroups and win where the runs are short.
+        // Measured (FP64 / FP32 ms per launch): 256^3 march 0.656 / 0.495, tiles 0.805 /
+        // 0.508; 128^3 march 0.164 / 0.118, tiles 0.125 / 0.084; 64^3 march 0.143 / 0.105,
+        // tiles 0.036 / 0.039.  FP32 marches with 6 waves per workgroup (3 per SIMD, 5 %
+        // faster); FP64 with 4 (6 measured 60 % slower).  MFLBM_GRAD_SHAPE (0-4 march shapes,
+        // 5 tiles) and MFLBM_GRAD_KCHUNK (march z-run) override the choice (tuning).
+        const char* shape_env = std::getenv("MFLBM_GRAD_SHAPE");
+        const char* kc_env = std::getenv("MFLBM_GRAD_KCHUNK");
         h->grad_shape = p.precision == MFLBM_FP32 ? GRAD_SHAPE_FP32 : GRAD_SHAPE_DEFAULT;
-        if (const char* env = std::getenv("MFLBM_GRAD_SHAPE")) h->grad_shape = std::atoi(env);
-        if (h->grad_shape < 0 || h->grad_shape >= GRAD_NSHAPES) return fail(h, "MFLBM_GRAD_SHAPE out of range");
-        const GradShape gs = GRAD_SHAPES[h->grad_shape];
+        if (shape_env) h->grad_shape = std::atoi(shape_env);
+        if (h->grad_shape < 0 || h->grad_shape > GRAD_SHAPE_TILES) return fail(h, "MFLBM_GRAD_SHAPE out of range");
+        const GradShape gs = GRAD_SHAPES[h->grad_shape == GRAD_SHAPE_TILES ? 0 : h->grad_shape];
         h->tiles_x = (d.nx + gs.tx - 1) / gs.tx;
         h->tiles_y = (d.ny + gs.ty - 1) / gs.ty;
-        // z-run per workgroup: long enough to amortise the 8 halo planes, short enough to
-        // give each of the 256 CUs two workgroups (the LDS use allows two);
-        // MFLBM_GRAD_KCHUNK and MFLBM_GRAD_SHAPE override (tuning)
+        // z-run per marching workgroup: long enough to amortise the 8 halo planes, short
+        // enough to give each of the 256 CUs two workgroups (the LDS use allows two)
         const int tiles = h->tiles_x * h->tiles_y;
         int kc = std::max(16, (d.nz * tiles + 511) / 512);
-        if (const char* env = std::getenv("MFLBM_GRAD_KCHUNK")) kc = std::max(1, std::atoi(env));
+        if (kc_env) kc = std::max(1, std::atoi(kc_env));
+        if (!shape_env && !kc_env && kc < GRAD_MARCH_MIN_RUN) h->grad_shape = GRAD_SHAPE_TILES;
         h->kchunk = std::min(kc, d.nz);
     }
 
@@ -852,6 +861,7 @@ int mflbm_sync(mflbm_handle* h) {
 
 int64_t mflbm_count(const mflbm_handle* h, int32_t which) {
     if (!h) return -1;
+    if (which == 3) return h->split_gradient ? GRAD_SHAPE_TILES + 1 : h->grad_shape;
     return which == 0 ? h->n_fluid : which == 1 ? h->n_fluid_ext : which == 2 ? h->n_sb : -1;
 }
 

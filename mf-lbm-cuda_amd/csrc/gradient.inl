@@ -377,5 +377,294 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Two-pass chain on 3-D tiles.  The z-march above advances one plane per pass of its five
+// dependent stages (about 5 us a plane at two workgroups per CU whatever the tile area),
+// so its time is set by latency, not by work or bandwidth.  Here the chain is cut where
+// the normals are complete: k_grad_normals computes K3 and K4+K5 on a box of nodes and
+// writes the normals and c_norm of its fluid nodes to the dense cn arrays; k_grad_force
+// reads them back, extrapolates them to the solid-boundary nodes (K6) and computes the
+// curvature and force records (K7).  Each box is loaded once with a halo of two nodes,
+// every workgroup runs three short phases, and many small workgroups per CU hide the
+// latency.  Node sets and arithmetic are those of the march: K3 over [-2, n+3], K4 over
+// [-1, n+2], K6 over [0, n+1], K7 over [1, n], through the same device functions.
+
+// Stage lists of a box, built from the tag words each thread loaded: a wave counts its
+// entries with ballots and reserves its runs with one LDS atomic on the packed counts.
+struct BoxLists {
+    template <int PER, class F0, class F1, class W0, class W1>
+    __device__ __forceinline__ static void build(unsigned* cnt, int lane, F0&& f0, F1&& f1, W0&& w0, W1&& w1) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int m = 0; m < PER; ++m)
+            c += (uint32_t)__popcll(__ballot(f0(m))) | (uint32_t)__popcll(__ballot(f1(m))) << 16;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cnt, c);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        int b0 = (int)(base & 0xFFFF), b1 = (int)(base >> 16);
+        auto below = [](uint64_t mk) {
+            return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
+        };
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const bool a0 = f0(m), a1 = f1(m);
+            const uint64_t m0 = __ballot(a0), m1 = __ballot(a1);
+            if (a0) w0(m, b0 + below(m0));
+            if (a1) w1(m, b1 + below(m1));
+            b0 += __popcll(m0); b1 += __popcll(m1);
+        }
+    }
+};
+
+// Tile of owned nodes (i0, j0, k0) + [0, TX) x [0, TY) x [0, TZ) from the block index,
+// tiles laid out x fastest over `nt` tiles per axis starting at `org`.
+__device__ __forceinline__ void box_origin(int tx_n, int ty_n, int org, int TX, int TY, int TZ, int& i0, int& j0,
+                                           int& k0) {
+    const int b = xcd_block();
+    i0 = org + (b % tx_n) * TX;
+    j0 = org + ((b / tx_n) % ty_n) * TY;
+    k0 = org + (b / (tx_n * ty_n)) * TZ;
+}
+
+// K3 + K4/K5 on the box of owned nodes inside [-1, n+2]^3: phi of the solid-boundary nodes
+// of the box and its 1-halo, then normals and c_norm of the fluid nodes of the box.
+template <typename T, int TX, int TY, int TZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepConst<T> P) {
+    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NL = LX * LY * LZ;
+    constexpr int PER = (NL + THREADS - 1) / THREADS;
+    constexpr int N3 = (TX + 2) * (TY + 2) * (TZ + 2), N4 = TX * TY * TZ;
+    static_assert(NL <= 4096, "list entries keep the box position in 12 bits");
+    __shared__ T s_phi[NL];
+    __shared__ uint32_t s_l3[N3];  // e | fluid-neighbour mask << 12
+    __shared__ uint16_t s_l4[N4];  // e | wet << 12
+    __shared__ unsigned s_cnt;     // K3 count | K4 count << 16
+    __shared__ T s_ws[7][13];
+    const Dims& d = A.d;
+    const int tid = threadIdx.x, lane = tid & 63;
+    int i0, j0, k0;
+    box_origin(A.tiles_x, A.tiles_y, -1, TX, TY, TZ, i0, j0, k0);
+    // owned extent, clipped to [-1, n+2]
+    const int ox = min(TX, d.nx + 3 - i0), oy = min(TY, d.ny + 3 - j0), oz = min(TZ, d.nz + 3 - k0);
+    if (tid < 7 * 13) {
+        const int a = tid / 13, dd = tid % 13;
+        T ws = T(0);
+        for (int m = 0; m < a; ++m) ws += W<T>::w1;
+        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
+        s_ws[a][dd] = ws;
+    }
+    if (tid == 0) s_cnt = 0;
+
+    T pv[PER];
+    int32_t tv[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int e = tid + m * THREADS;
+        const int x = e % LX, y = (e / LX) % LY, z = e / (LX * LY);
+        const int gi = i0 - 2 + x, gj = j0 - 2 + y, gk = k0 - 2 + z;
+        pv[m] = T(0);
+        tv[m] = TAG_SOLID;
+        if (e < NL && gi <= d.nx + 4 && gj <= d.ny + 4 && gk <= d.nz + 4) {
+            const int c = i4(d, gi, gj, gk);
+            pv[m] = A.phi[c];
+            tv[m] = A.tag[c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int e = tid + m * THREADS;
+        if (e < NL) s_phi[e] = pv[m];
+    }
+    auto in_box = [&](int e, int h) {
+        const int x = e % LX - 2, y = (e / LX) % LY - 2, z = e / (LX * LY) - 2;
+        return x >= -h && x < ox + h && y >= -h && y < oy + h && z >= -h && z < oz + h;
+    };
+    BoxLists::build<PER>(
+        &s_cnt, lane,
+        [&](int m) { const int e = tid + m * THREADS; return e < NL && (tv[m] & 3) == TAG_SOLID_BND && in_box(e, 1); },
+        [&](int m) { const int e = tid + m * THREADS; return e < NL && tag_fluid(tv[m] & 3) && in_box(e, 0); },
+        [&](int m, int at) { s_l3[at] = (uint32_t)(tid + m * THREADS) | ((uint32_t)tv[m] >> 2) << 12; },
+        [&](int m, int at) { s_l4[at] = (uint16_t)((tid + m * THREADS) | ((tv[m] & 3) == TAG_WET) << 12); });
+    __syncthreads();
+    const int n3 = min((int)(s_cnt & 0xFFFF), N3), n4 = min((int)(s_cnt >> 16), N4);
+
+    // the solid normals of the wetting nodes among this thread's K4 entries, issued now so
+    // that their latency hides behind K3
+    constexpr int M4 = (N4 + THREADS - 1) / THREADS;
+    T wn[M4][3];
+#pragma unroll
+    for (int m = 0; m < M4; ++m) {
+        const int i = tid + m * THREADS;
+        wn[m][0] = wn[m][1] = wn[m][2] = T(0);
+        if (i < n4) {
+            const int v = s_l4[i], e = v & 0xFFF;
+            if ((v >> 12) & 1) {
+                const int c = i4(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / (LX * LY));
+                wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
+            }
+        }
+    }
+
+    // K3 (src/main_iteration_GPU.cu:732-755), in place: reads fluid entries, writes
+    // solid-boundary ones; the solid-boundary phi of [1, nx] x [1, ny] x [0, nz+1] owned by
+    // this box goes back to memory (the later kernels of the step read it there)
+    for (int i = tid; i < n3; i += THREADS) {
+        const uint32_t lv = s_l3[i];
+        const int e = (int)(lv & 0xFFF);
+        const uint32_t mask = lv >> 12;
+        T ps = T(0);
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            const bool f = (mask >> (q - 1)) & 1u;
+            ps += keep_if(f, s_phi[e + EX[q] + LX * (EY[q] + LY * EZ[q])]) * wq<T>(q);
+        }
+        const T v = ps / s_ws[__popc(mask & 63u)][__popc(mask >> 6)];
+        s_phi[e] = v;
+        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
+        if (in_box(e, 0) && gi >= 1 && gi <= d.nx && gj >= 1 && gj <= d.ny && gk >= 0 && gk <= d.nz + 1)
+            A.phi[i4(d, gi, gj, gk)] = v;
+    }
+    __syncthreads();
+
+    // K4 + K5 (src/main_iteration_GPU.cu:757-878)
+#pragma unroll
+    for (int m = 0; m < M4; ++m) {
+        const int i = tid + m * THREADS;
+        if (i >= n4) break;
+        const int v = s_l4[i], e = v & 0xFFF;
+        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
+        const bool wet = (v >> 12) & 1;
+        auto a = [&](int dx, int dy, int dz) -> T { return s_phi[e + dx + LX * (dy + LY * dz)]; };
+        const T cx = iso4_at<T, 0>(a);
+        const T cy = iso4_at<T, 1>(a);
+        const T cz = iso4_at<T, 2>(a);
+        T vx, vy, vz;
+        const T cnorm = interface_normal<T>(
+            cx, cy, cz, wet, P.cos_theta, [&](T& nwx, T& nwy, T& nwz) { nwx = wn[m][0]; nwy = wn[m][1]; nwz = wn[m][2]; },
+            vx, vy, vz);
+        const int c2 = i2(d, gi, gj, gk);
+        A.cnx[c2] = vx; A.cny[c2] = vy; A.cnz[c2] = vz; A.cnorm[c2] = cnorm;
+    }
+}
+
+// K6 + K7 on the box of owned nodes inside [1, n]^3: normals of the solid-boundary nodes of
+// the box and its 1-halo from the fluid normals around them, then the curvature and the
+// force record of the fluid nodes of the box.
+template <typename T, int TX, int TY, int TZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_grad_force(GradArgs<T> A, StepConst<T> P) {
+    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NL = LX * LY * LZ;
+    constexpr int PER = (NL + THREADS - 1) / THREADS;
+    constexpr int N6 = (TX + 2) * (TY + 2) * (TZ + 2), N7 = TX * TY * TZ;
+    static_assert(NL <= 4096, "list entries keep the box position in 12 bits");
+    __shared__ T s_cn[3][NL];
+    __shared__ uint32_t s_l6[N6];  // e | fluid-neighbour mask << 12
+    __shared__ uint16_t s_l7[N7];  // e
+    __shared__ int32_t s_r7[N7];   // force-record index
+    __shared__ unsigned s_cnt;     // K6 count | K7 count << 16
+    __shared__ T s_ws[7][13];
+    const Dims& d = A.d;
+    const int tid = threadIdx.x, lane = tid & 63;
+    int i0, j0, k0;
+    box_origin(A.tiles_x, A.tiles_y, 1, TX, TY, TZ, i0, j0, k0);
+    const int ox = min(TX, d.nx + 1 - i0), oy = min(TY, d.ny + 1 - j0), oz = min(TZ, d.nz + 1 - k0);
+    if (tid < 7 * 13) {
+        const int a = tid / 13, dd = tid % 13;
+        T ws = T(0);
+        for (int m = 0; m < a; ++m) ws += W<T>::w1;
+        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
+        s_ws[a][dd] = ws;
+    }
+    if (tid == 0) s_cnt = 0;
+
+    // the fluid normals of the box with its 2-halo (inside [-1, n+2], where K4 wrote them)
+    int32_t tv[PER];
+    T cv[PER][3];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int e = tid + m * THREADS;
+        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
+        tv[m] = TAG_SOLID;
+        cv[m][0] = cv[m][1] = cv[m][2] = T(0);
+        // the normals are loaded whatever the node type (only fluid entries are read before
+        // K6 writes the solid-boundary ones), so that no load waits for the tag
+        if (e < NL && gi <= d.nx + 2 && gj <= d.ny + 2 && gk <= d.nz + 2) {
+            const int c2 = i2(d, gi, gj, gk);
+            tv[m] = A.tag[i4(d, gi, gj, gk)];
+            cv[m][0] = A.cnx[c2]; cv[m][1] = A.cny[c2]; cv[m][2] = A.cnz[c2];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int e = tid + m * THREADS;
+        if (e < NL) { s_cn[0][e] = cv[m][0]; s_cn[1][e] = cv[m][1]; s_cn[2][e] = cv[m][2]; }
+    }
+    auto in_box = [&](int e, int h) {
+        const int x = e % LX - 2, y = (e / LX) % LY - 2, z = e / (LX * LY) - 2;
+        return x >= -h && x < ox + h && y >= -h && y < oy + h && z >= -h && z < oz + h;
+    };
+    BoxLists::build<PER>(
+        &s_cnt, lane,
+        [&](int m) { const int e = tid + m * THREADS; return e < NL && (tv[m] & 3) == TAG_SOLID_BND && in_box(e, 1); },
+        [&](int m) { const int e = tid + m * THREADS; return e < NL && tag_fluid(tv[m] & 3) && in_box(e, 0); },
+        [&](int m, int at) { s_l6[at] = (uint32_t)(tid + m * THREADS) | ((uint32_t)tv[m] >> 2) << 12; },
+        [&](int m, int at) { s_l7[at] = (uint16_t)(tid + m * THREADS); s_r7[at] = tv[m] >> 2; });
+    __syncthreads();
+    const int n6 = min((int)(s_cnt & 0xFFFF), N6), n7 = min((int)(s_cnt >> 16), N7);
+
+    // c_norm of this thread's K7 entries, issued now so that its latency hides behind K6
+    constexpr int M7 = (N7 + THREADS - 1) / THREADS;
+    T cnv[M7];
+#pragma unroll
+    for (int m = 0; m < M7; ++m) {
+        const int i = tid + m * THREADS;
+        cnv[m] = T(0);
+        if (i < n7) {
+            const int e = s_l7[i];
+            cnv[m] = A.cnorm[i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / (LX * LY))];
+        }
+    }
+
+    // K6 (src/main_iteration_GPU.cu:880-906), in place like K3
+    for (int i = tid; i < n6; i += THREADS) {
+        const uint32_t lv = s_l6[i];
+        const int e = (int)(lv & 0xFFF);
+        const uint32_t mask = lv >> 12;
+        T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            const bool f = (mask >> (q - 1)) & 1u;
+            const int o = e + EX[q] + LX * (EY[q] + LY * EZ[q]);
+            sx += keep_if(f, s_cn[0][o]) * wq<T>(q);
+            sy += keep_if(f, s_cn[1][o]) * wq<T>(q);
+            sz += keep_if(f, s_cn[2][o]) * wq<T>(q);
+        }
+        const T ws = s_ws[__popc(mask & 63u)][__popc(mask >> 6)];
+        s_cn[0][e] = sx / ws; s_cn[1][e] = sy / ws; s_cn[2][e] = sz / ws;
+    }
+    __syncthreads();
+
+    // K7 (src/main_iteration_GPU.cu:908-1003): away from interfaces c_norm and the normals
+    // are zero and so is the record, whatever the curvature
+#pragma unroll
+    for (int m = 0; m < M7; ++m) {
+        const int i = tid + m * THREADS;
+        if (i >= n7) break;
+        const int e = s_l7[i];
+        const T c_norm = cnv[m];
+        auto comp = [&](int c) {
+            return [&, c](int dx, int dy, int dz) -> T { return s_cn[c][e + dx + LX * (dy + LY * dz)]; };
+        };
+        T curv = T(0);
+        if (c_norm != T(0)) curv = curvature_of<T>(comp(0), comp(1), comp(2));
+        T* a = A.aux + 4 * (int64_t)s_r7[i];
+        a[0] = s_cn[0][e];
+        a[1] = s_cn[1][e];
+        a[2] = s_cn[2][e];
+        a[3] = T(0.5) * P.gamma * curv * c_norm;
+    }
+}
+
 }  // namespace dev
 }  // namespace mflbm

@@ -136,6 +136,10 @@ struct GradArgs {
     const T* __restrict__ sny;
     const T* __restrict__ snz;
     T* aux;                           // 4 * nf force records
+    T* cnx;                           // S2 dense normals and c_norm (two-pass chain)
+    T* cny;
+    T* cnz;
+    T* cnorm;
     int shape, tiles_x, tiles_y, kchunk;
 };
 
@@ -146,6 +150,12 @@ struct GradShape {
 constexpr GradShape GRAD_SHAPES[] = {{16, 16, 256}, {16, 8, 256}, {8, 8, 128}, {16, 16, 384}, {32, 16, 512}};
 constexpr int GRAD_NSHAPES = sizeof(GRAD_SHAPES) / sizeof(GRAD_SHAPES[0]);
 constexpr int GRAD_SHAPE_DEFAULT = 0, GRAD_SHAPE_FP32 = 3;
+// the two-pass chain on 3-D tiles (k_grad_normals, k_grad_force): the shape after the
+// march shapes
+constexpr int GRAD_SHAPE_TILES = GRAD_NSHAPES;
+constexpr int GRAD_TILE_N[3] = {16, 8, 8}, GRAD_TILE_F[3] = {8, 8, 8};
+// shortest z-run for which the march is chosen by default
+constexpr int GRAD_MARCH_MIN_RUN = 64;
 
 // Uniform deviate in [0, 1) of lattice node n under a seed, for the random initial phase
 // distribution: the splitmix64 output function (Steele, Lea and Flood, OOPSLA 2014) of

@@ -78,6 +78,21 @@ void Launch<T>::curvature(const Fields<T>& F, const StepConst<T>& P, const NodeL
 
 template <typename T>
 void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_t s) {
+    if (A.shape == GRAD_SHAPE_TILES) {
+        constexpr int NX = GRAD_TILE_N[0], NY = GRAD_TILE_N[1], NZ = GRAD_TILE_N[2];
+        constexpr int FX = GRAD_TILE_F[0], FY = GRAD_TILE_F[1], FZ = GRAD_TILE_F[2];
+        GradArgs<T> B = A;
+        // normals over [-1, n+2]^3, forces over [1, n]^3
+        B.tiles_x = (A.d.nx + 4 + NX - 1) / NX;
+        B.tiles_y = (A.d.ny + 4 + NY - 1) / NY;
+        const unsigned gn = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + 4 + NZ - 1) / NZ));
+        dev::k_grad_normals<T, NX, NY, NZ, 256><<<gn, 256, 0, s>>>(B, P);
+        B.tiles_x = (A.d.nx + FX - 1) / FX;
+        B.tiles_y = (A.d.ny + FY - 1) / FY;
+        const unsigned gf = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + FZ - 1) / FZ));
+        dev::k_grad_force<T, FX, FY, FZ, 256><<<gf, 256, 0, s>>>(B, P);
+        return;
+    }
     const int chunks = (A.d.nz + A.kchunk - 1) / A.kchunk;
     const unsigned grid = (unsigned)(A.tiles_x * A.tiles_y * chunks);
     static_assert(GRAD_NSHAPES == 5, "one case per shape");

@@ -273,3 +273,8 @@ class Solver:
         L = load()
         return {"fluid": L.mflbm_count(self.h, 0), "fluid_ext": L.mflbm_count(self.h, 1),
                 "solid_boundary": L.mflbm_count(self.h, 2)}
+
+    def gradient_path(self):
+        """'march' (one-pass z-march), 'tiles' (two-pass 3-D tiles) or 'split' (four kernels)"""
+        k = load().mflbm_count(self.h, 3)
+        return "split" if k == 6 else "tiles" if k == 5 else "march"

@@ -5,7 +5,11 @@ population slots including ghosts, phi, normals, curvature at fluid nodes) — t
 required bound is a relative L-infinity of 1e-12, bit equality is stronger.  Monitor
 sums are reduced in a different order on the device; they must agree to 1e-12 of the
 summed magnitudes.
+
+Every test runs on both gradient chains the library can choose: the one-pass z-march
+(the default on large domains) and the two-pass 3-D tiles (the default on small ones).
 """
+import os
 import sys
 
 import numpy as np
@@ -23,9 +27,22 @@ def say(msg):
     sys.stdout.flush()
 
 
+PATH = {"march": "0", "tiles": "5"}
+_chain = {}
+
+
+@pytest.fixture(autouse=True, params=sorted(PATH))
+def gradient_chain(request, monkeypatch):
+    """Forces the handle's gradient chain (MFLBM_GRAD_SHAPE is read at creation)."""
+    monkeypatch.setenv("MFLBM_GRAD_SHAPE", PATH[request.param])
+    _chain["name"] = request.param
+    yield request.param
+
+
 def pair(geo, params, precision=gpu.FP64):
     o = O.Oracle(params, geo)
     g = gpu.Solver(params, geo, precision)
+    assert g.gradient_path() == _chain["name"]
     return o, g
 
 
@@ -180,8 +197,10 @@ def test_random_initial_distribution_bitwise(open_z):
     compare_state(o, g, tag="step 30")
 
 
-def test_fused_gradient_short_z_runs(monkeypatch):
+def test_fused_gradient_short_z_runs(monkeypatch, gradient_chain):
     """Workgroups marching 5-plane z-runs (many run boundaries) still match the oracle."""
+    if gradient_chain != "march":
+        pytest.skip("z-runs belong to the march")
     monkeypatch.setenv("MFLBM_GRAD_KCHUNK", "5")
     geo, p = CASES["drainage_ywalls"]()
     say("case=drainage_ywalls kchunk=5, 24 steps")
