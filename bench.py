@@ -112,7 +112,8 @@ def kernel_group(name):
         return "probe"
     for key, short in (("k_collide<double, true>", "collide_odd"), ("k_collide<float, true>", "collide_odd"),
                        ("k_collide<double, false>", "collide_even"), ("k_collide<float, false>", "collide_even"),
-                       ("k_gradient<", "gradient")):
+                       ("k_gradient<", "gradient"), ("k_grad_normals<", "grad_normals"),
+                       ("k_grad_force<", "grad_force")):
         if key in name:
             return short
     return None
@@ -169,7 +170,7 @@ def counter_pass(counters, geo_path, args, timeout=240):
     return vals
 
 
-GROUPS = ("collide_odd", "collide_even", "gradient", "probe", "probe_blocked")
+GROUPS = ("collide_odd", "collide_even", "gradient", "grad_normals", "grad_force", "probe", "probe_blocked")
 
 
 def measure_traffic(geo, args):
@@ -213,6 +214,8 @@ def measure_traffic(geo, args):
             res["fetch_write"][g] = {"fetch": round(fb), "write": round(wb), "total": round(fb + wb)}
     for kind in ("requests", "fetch_write"):
         d = res[kind]
+        if "gradient" not in d and "grad_normals" in d and "grad_force" in d:  # the two-pass chain
+            d["gradient"] = {k: d["grad_normals"][k] + d["grad_force"][k] for k in ("fetch", "write", "total")}
         log(f"traffic per dispatch ({kind}, GB): " + ", ".join(
             f"{g} {d[g]['fetch'] / 1e9:.3f}+{d[g]['write'] / 1e9:.3f}" for g in GROUPS if g in d))
     return res
@@ -345,6 +348,7 @@ def main():
         pms = sim.bandwidth_probe(nbytes, repeats=10)
         probe = {"bytes": nbytes, "ms": round(pms, 4), "gbs": round(2 * nbytes / (pms * 1e-3) / 1e9, 1)}
         log(f"plain copy of {args.probe_mib} MiB: {pms:.3f} ms = {probe['gbs']} GB/s (read+write)")
+    chain = sim.gradient_path()
     sim.close()
 
     cpu = None
@@ -374,6 +378,7 @@ def main():
                           "frac": round(step_frac, 4),
                           "definition": "(N_fluid * 78 * s + N_all * 1 B) / timed ms_per_step / 8 TB/s"},
         "fluid_glups": round(n_fluid * args.steps / dt / 1e9, 4),
+        "gradient_chain": chain,
         "kernel_ms": {"collide_odd": round(odd_ms, 4), "collide_even": round(even_ms, 4),
                       "boundary": round(bnd_ms, 4), "gradient": round(grad_ms, 4), "step": round(step_ms, 4)},
         "late": late,

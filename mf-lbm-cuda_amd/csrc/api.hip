@@ -505,12 +505,13 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
         // Gradient chain.  The z-march (one pass, LDS rings) advances one plane per ~5 us
         // whatever the tile area, so it pays where its z-runs are long; the two-pass 3-D
-        // tiles launch thousands of small workgroups and win where the runs are short.
-        // Measured (FP64 / FP32 ms per launch): 256^3 march 0.656 / 0.495, tiles 0.805 /
-        // 0.508; 128^3 march 0.164 / 0.118, tiles 0.125 / 0.084; 64^3 march 0.143 / 0.105,
-        // tiles 0.036 / 0.039.  FP32 marches with 6 waves per workgroup (3 per SIMD, 5 %
-        // faster); FP64 with 4 (6 measured 60 % slower).  MFLBM_GRAD_SHAPE (0-4 march shapes,
-        // 5 tiles) and MFLBM_GRAD_KCHUNK (march z-run) override the choice (tuning).
+        // tiles launch thousands of small workgroups and win where the runs are short, and
+        // in FP32, where their extra arithmetic (halo recomputation in three dimensions)
+        // costs less.  Measured (FP64 / FP32 ms per launch): 256^3 march 0.651 / 0.495,
+        // tiles 0.816 / 0.427; 64^3 march 0.143 / 0.105, tiles 0.026 / 0.021.  FP32 marches
+        // with 6 waves per workgroup (3 per SIMD, 5 % faster); FP64 with 4 (6 measured 60 %
+        // slower).  MFLBM_GRAD_SHAPE (0-4 march shapes, 5 tiles) and MFLBM_GRAD_KCHUNK
+        // (march z-run) override the choice (tuning).
         const char* shape_env = std::getenv("MFLBM_GRAD_SHAPE");
         const char* kc_env = std::getenv("MFLBM_GRAD_KCHUNK");
         h->grad_shape = p.precision == MFLBM_FP32 ? GRAD_SHAPE_FP32 : GRAD_SHAPE_DEFAULT;
@@ -524,7 +525,8 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         const int tiles = h->tiles_x * h->tiles_y;
         int kc = std::max(16, (d.nz * tiles + 511) / 512);
         if (kc_env) kc = std::max(1, std::atoi(kc_env));
-        if (!shape_env && !kc_env && kc < GRAD_MARCH_MIN_RUN) h->grad_shape = GRAD_SHAPE_TILES;
+        if (!shape_env && !kc_env && (p.precision == MFLBM_FP32 || kc < GRAD_MARCH_MIN_RUN))
+            h->grad_shape = GRAD_SHAPE_TILES;
         h->kchunk = std::min(kc, d.nz);
     }
 

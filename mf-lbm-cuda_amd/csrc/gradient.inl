@@ -381,7 +381,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
 // ---------------------------------------------------------------------------------------
 // Two-pass chain on 3-D tiles.  The z-march above advances one plane per pass of its five
 // dependent stages (about 5 us a plane at two workgroups per CU whatever the tile area),
-// so its time is set by latency, not by work or bandwidth.  Here the chain is cut where
+// so its time is set by latency where its z-runs are short.  Here the chain is cut where
 // the normals are complete: k_grad_normals computes K3 and K4+K5 on a box of nodes and
 // writes the normals and c_norm of its fluid nodes to the dense cn arrays; k_grad_force
 // reads them back, extrapolates them to the solid-boundary nodes (K6) and computes the
@@ -389,6 +389,10 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
 // every workgroup runs three short phases, and many small workgroups per CU hide the
 // latency.  Node sets and arithmetic are those of the march: K3 over [-2, n+3], K4 over
 // [-1, n+2], K6 over [0, n+1], K7 over [1, n], through the same device functions.
+//
+// A workgroup has one thread per column (x, y) of its box (LX * LY threads); the thread
+// walks its column up z, so that its loads, LDS stores and list flags take one address
+// step per plane.  Box entry e = x + LX * (y + LY * z).
 
 // Stage lists of a box, built from the tag words each thread loaded: a wave counts its
 // entries with ballots and reserves its runs with one LDS atomic on the packed counts.
@@ -417,8 +421,8 @@ struct BoxLists {
     }
 };
 
-// Tile of owned nodes (i0, j0, k0) + [0, TX) x [0, TY) x [0, TZ) from the block index,
-// tiles laid out x fastest over `nt` tiles per axis starting at `org`.
+// Owned box (i0, j0, k0) + [0, TX) x [0, TY) x [0, TZ) of this workgroup: boxes laid out x
+// fastest, `tx_n` x `ty_n` per z layer, the first at (org, org, org).
 __device__ __forceinline__ void box_origin(int tx_n, int ty_n, int org, int TX, int TY, int TZ, int& i0, int& j0,
                                            int& k0) {
     const int b = xcd_block();
@@ -427,12 +431,22 @@ __device__ __forceinline__ void box_origin(int tx_n, int ty_n, int org, int TX, 
     k0 = org + (b / (tx_n * ty_n)) * TZ;
 }
 
+template <typename T>
+__device__ __forceinline__ void ws_table(T (*s_ws)[13], int tid) {
+    if (tid < 7 * 13) {
+        const int a = tid / 13, dd = tid % 13;
+        T ws = T(0);
+        for (int m = 0; m < a; ++m) ws += W<T>::w1;
+        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
+        s_ws[a][dd] = ws;
+    }
+}
+
 // K3 + K4/K5 on the box of owned nodes inside [-1, n+2]^3: phi of the solid-boundary nodes
 // of the box and its 1-halo, then normals and c_norm of the fluid nodes of the box.
-template <typename T, int TX, int TY, int TZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepConst<T> P) {
-    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NL = LX * LY * LZ;
-    constexpr int PER = (NL + THREADS - 1) / THREADS;
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_normals(GradArgs<T> A, StepConst<T> P) {
+    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NC = LX * LY, NL = NC * LZ, THREADS = NC;
     constexpr int N3 = (TX + 2) * (TY + 2) * (TZ + 2), N4 = TX * TY * TZ;
     static_assert(NL <= 4096, "list entries keep the box position in 12 bits");
     __shared__ T s_phi[NL];
@@ -446,46 +460,35 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
     box_origin(A.tiles_x, A.tiles_y, -1, TX, TY, TZ, i0, j0, k0);
     // owned extent, clipped to [-1, n+2]
     const int ox = min(TX, d.nx + 3 - i0), oy = min(TY, d.ny + 3 - j0), oz = min(TZ, d.nz + 3 - k0);
-    if (tid < 7 * 13) {
-        const int a = tid / 13, dd = tid % 13;
-        T ws = T(0);
-        for (int m = 0; m < a; ++m) ws += W<T>::w1;
-        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
-        s_ws[a][dd] = ws;
-    }
+    ws_table<T>(s_ws, tid);
     if (tid == 0) s_cnt = 0;
 
-    T pv[PER];
-    int32_t tv[PER];
+    // this thread's column; entries beyond n+4 load a clamped address and count as solid
+    const int x = tid % LX, y = tid / LX;
+    const int gi = i0 - 2 + x, gj = j0 - 2 + y;
+    const bool col = gi <= d.nx + 4 && gj <= d.ny + 4;
+    const int64_t P4 = (int64_t)d.X4 * d.Y4;
+    const int64_t c0 = i4(d, min(gi, d.nx + 4), min(gj, d.ny + 4), k0 - 2);
+    const int zmax = d.nz + 4 - (k0 - 2);  // last loadable z of the box
+    T pv[LZ];
+    int32_t tv[LZ];
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = tid + m * THREADS;
-        const int x = e % LX, y = (e / LX) % LY, z = e / (LX * LY);
-        const int gi = i0 - 2 + x, gj = j0 - 2 + y, gk = k0 - 2 + z;
-        pv[m] = T(0);
-        tv[m] = TAG_SOLID;
-        if (e < NL && gi <= d.nx + 4 && gj <= d.ny + 4 && gk <= d.nz + 4) {
-            const int c = i4(d, gi, gj, gk);
-            pv[m] = A.phi[c];
-            tv[m] = A.tag[c];
-        }
+    for (int z = 0; z < LZ; ++z) {
+        const int64_t c = c0 + (int64_t)min(z, zmax) * P4;
+        pv[z] = A.phi[c];
+        tv[z] = A.tag[c];
+        if (!col || z > zmax) { pv[z] = T(0); tv[z] = TAG_SOLID; }
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = tid + m * THREADS;
-        if (e < NL) s_phi[e] = pv[m];
-    }
-    auto in_box = [&](int e, int h) {
-        const int x = e % LX - 2, y = (e / LX) % LY - 2, z = e / (LX * LY) - 2;
-        return x >= -h && x < ox + h && y >= -h && y < oy + h && z >= -h && z < oz + h;
-    };
-    BoxLists::build<PER>(
-        &s_cnt, lane,
-        [&](int m) { const int e = tid + m * THREADS; return e < NL && (tv[m] & 3) == TAG_SOLID_BND && in_box(e, 1); },
-        [&](int m) { const int e = tid + m * THREADS; return e < NL && tag_fluid(tv[m] & 3) && in_box(e, 0); },
-        [&](int m, int at) { s_l3[at] = (uint32_t)(tid + m * THREADS) | ((uint32_t)tv[m] >> 2) << 12; },
-        [&](int m, int at) { s_l4[at] = (uint16_t)((tid + m * THREADS) | ((tv[m] & 3) == TAG_WET) << 12); });
+    for (int z = 0; z < LZ; ++z) s_phi[tid + NC * z] = pv[z];
+    const bool c3 = x >= 1 && x < ox + 3 && y >= 1 && y < oy + 3;  // 1-halo column
+    const bool c4 = x >= 2 && x < ox + 2 && y >= 2 && y < oy + 2;  // owned column
+    BoxLists::build<LZ>(
+        &s_cnt, lane, [&](int z) { return c3 && z >= 1 && z < oz + 3 && (tv[z] & 3) == TAG_SOLID_BND; },
+        [&](int z) { return c4 && z >= 2 && z < oz + 2 && tag_fluid(tv[z] & 3); },
+        [&](int z, int at) { s_l3[at] = (uint32_t)(tid + NC * z) | ((uint32_t)tv[z] >> 2) << 12; },
+        [&](int z, int at) { s_l4[at] = (uint16_t)((tid + NC * z) | ((tv[z] & 3) == TAG_WET) << 12); });
     __syncthreads();
     const int n3 = min((int)(s_cnt & 0xFFFF), N3), n4 = min((int)(s_cnt >> 16), N4);
 
@@ -500,7 +503,7 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
         if (i < n4) {
             const int v = s_l4[i], e = v & 0xFFF;
             if ((v >> 12) & 1) {
-                const int c = i4(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / (LX * LY));
+                const int c = i4(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC);
                 wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
             }
         }
@@ -521,9 +524,11 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
         }
         const T v = ps / s_ws[__popc(mask & 63u)][__popc(mask >> 6)];
         s_phi[e] = v;
-        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
-        if (in_box(e, 0) && gi >= 1 && gi <= d.nx && gj >= 1 && gj <= d.ny && gk >= 0 && gk <= d.nz + 1)
-            A.phi[i4(d, gi, gj, gk)] = v;
+        const int bx = e % LX - 2, by = (e / LX) % LY - 2, bz = e / NC - 2;
+        const int qi = i0 + bx, qj = j0 + by, qk = k0 + bz;
+        if (bx >= 0 && bx < ox && by >= 0 && by < oy && bz >= 0 && bz < oz && qi >= 1 && qi <= d.nx && qj >= 1 &&
+            qj <= d.ny && qk >= 0 && qk <= d.nz + 1)
+            A.phi[i4(d, qi, qj, qk)] = v;
     }
     __syncthreads();
 
@@ -533,7 +538,6 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
         const int i = tid + m * THREADS;
         if (i >= n4) break;
         const int v = s_l4[i], e = v & 0xFFF;
-        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
         const bool wet = (v >> 12) & 1;
         auto a = [&](int dx, int dy, int dz) -> T { return s_phi[e + dx + LX * (dy + LY * dz)]; };
         const T cx = iso4_at<T, 0>(a);
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
         const T cnorm = interface_normal<T>(
             cx, cy, cz, wet, P.cos_theta, [&](T& nwx, T& nwy, T& nwz) { nwx = wn[m][0]; nwy = wn[m][1]; nwz = wn[m][2]; },
             vx, vy, vz);
-        const int c2 = i2(d, gi, gj, gk);
+        const int c2 = i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC);
         A.cnx[c2] = vx; A.cny[c2] = vy; A.cnz[c2] = vz; A.cnorm[c2] = cnorm;
     }
 }
@@ -551,10 +555,9 @@ __global__ __launch_bounds__(THREADS) void k_grad_normals(GradArgs<T> A, StepCon
 // K6 + K7 on the box of owned nodes inside [1, n]^3: normals of the solid-boundary nodes of
 // the box and its 1-halo from the fluid normals around them, then the curvature and the
 // force record of the fluid nodes of the box.
-template <typename T, int TX, int TY, int TZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_grad_force(GradArgs<T> A, StepConst<T> P) {
-    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NL = LX * LY * LZ;
-    constexpr int PER = (NL + THREADS - 1) / THREADS;
+template <typename T, int TX, int TY, int TZ>
+__global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_force(GradArgs<T> A, StepConst<T> P) {
+    constexpr int LX = TX + 4, LY = TY + 4, LZ = TZ + 4, NC = LX * LY, NL = NC * LZ, THREADS = NC;
     constexpr int N6 = (TX + 2) * (TY + 2) * (TZ + 2), N7 = TX * TY * TZ;
     static_assert(NL <= 4096, "list entries keep the box position in 12 bits");
     __shared__ T s_cn[3][NL];
@@ -568,48 +571,44 @@ __global__ __launch_bounds__(THREADS) void k_grad_force(GradArgs<T> A, StepConst
     int i0, j0, k0;
     box_origin(A.tiles_x, A.tiles_y, 1, TX, TY, TZ, i0, j0, k0);
     const int ox = min(TX, d.nx + 1 - i0), oy = min(TY, d.ny + 1 - j0), oz = min(TZ, d.nz + 1 - k0);
-    if (tid < 7 * 13) {
-        const int a = tid / 13, dd = tid % 13;
-        T ws = T(0);
-        for (int m = 0; m < a; ++m) ws += W<T>::w1;
-        for (int m = 0; m < dd; ++m) ws += W<T>::w2;
-        s_ws[a][dd] = ws;
-    }
+    ws_table<T>(s_ws, tid);
     if (tid == 0) s_cnt = 0;
 
-    // the fluid normals of the box with its 2-halo (inside [-1, n+2], where K4 wrote them)
-    int32_t tv[PER];
-    T cv[PER][3];
+    // the fluid normals of the box with its 2-halo (inside [-1, n+2], where K4 wrote them),
+    // loaded whatever the node type (only fluid entries are read before K6 writes the
+    // solid-boundary ones) so that no load waits for the tag; entries beyond n+2 load a
+    // clamped address and count as solid
+    const int x = tid % LX, y = tid / LX;
+    const int gi = i0 - 2 + x, gj = j0 - 2 + y;
+    const bool col = gi <= d.nx + 2 && gj <= d.ny + 2;
+    const int64_t P4 = (int64_t)d.X4 * d.Y4, P2 = (int64_t)d.X2 * d.Y2;
+    const int ci = min(gi, d.nx + 2), cj = min(gj, d.ny + 2);
+    const int64_t t0 = i4(d, ci, cj, k0 - 2), n0 = i2(d, ci, cj, k0 - 2);
+    const int zmax = d.nz + 2 - (k0 - 2);
+    int32_t tv[LZ];
+    T cv[LZ][3];
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = tid + m * THREADS;
-        const int gi = i0 - 2 + e % LX, gj = j0 - 2 + (e / LX) % LY, gk = k0 - 2 + e / (LX * LY);
-        tv[m] = TAG_SOLID;
-        cv[m][0] = cv[m][1] = cv[m][2] = T(0);
-        // the normals are loaded whatever the node type (only fluid entries are read before
-        // K6 writes the solid-boundary ones), so that no load waits for the tag
-        if (e < NL && gi <= d.nx + 2 && gj <= d.ny + 2 && gk <= d.nz + 2) {
-            const int c2 = i2(d, gi, gj, gk);
-            tv[m] = A.tag[i4(d, gi, gj, gk)];
-            cv[m][0] = A.cnx[c2]; cv[m][1] = A.cny[c2]; cv[m][2] = A.cnz[c2];
-        }
+    for (int z = 0; z < LZ; ++z) {
+        const int zz = min(z, zmax);
+        const int64_t c2 = n0 + (int64_t)zz * P2;
+        tv[z] = A.tag[t0 + (int64_t)zz * P4];
+        cv[z][0] = A.cnx[c2]; cv[z][1] = A.cny[c2]; cv[z][2] = A.cnz[c2];
+        if (!col || z > zmax) tv[z] = TAG_SOLID;
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < PER; ++m) {
-        const int e = tid + m * THREADS;
-        if (e < NL) { s_cn[0][e] = cv[m][0]; s_cn[1][e] = cv[m][1]; s_cn[2][e] = cv[m][2]; }
+    for (int z = 0; z < LZ; ++z) {
+        s_cn[0][tid + NC * z] = cv[z][0];
+        s_cn[1][tid + NC * z] = cv[z][1];
+        s_cn[2][tid + NC * z] = cv[z][2];
     }
-    auto in_box = [&](int e, int h) {
-        const int x = e % LX - 2, y = (e / LX) % LY - 2, z = e / (LX * LY) - 2;
-        return x >= -h && x < ox + h && y >= -h && y < oy + h && z >= -h && z < oz + h;
-    };
-    BoxLists::build<PER>(
-        &s_cnt, lane,
-        [&](int m) { const int e = tid + m * THREADS; return e < NL && (tv[m] & 3) == TAG_SOLID_BND && in_box(e, 1); },
-        [&](int m) { const int e = tid + m * THREADS; return e < NL && tag_fluid(tv[m] & 3) && in_box(e, 0); },
-        [&](int m, int at) { s_l6[at] = (uint32_t)(tid + m * THREADS) | ((uint32_t)tv[m] >> 2) << 12; },
-        [&](int m, int at) { s_l7[at] = (uint16_t)(tid + m * THREADS); s_r7[at] = tv[m] >> 2; });
+    const bool c6 = x >= 1 && x < ox + 3 && y >= 1 && y < oy + 3;  // 1-halo column
+    const bool c7 = x >= 2 && x < ox + 2 && y >= 2 && y < oy + 2;  // owned column
+    BoxLists::build<LZ>(
+        &s_cnt, lane, [&](int z) { return c6 && z >= 1 && z < oz + 3 && (tv[z] & 3) == TAG_SOLID_BND; },
+        [&](int z) { return c7 && z >= 2 && z < oz + 2 && tag_fluid(tv[z] & 3); },
+        [&](int z, int at) { s_l6[at] = (uint32_t)(tid + NC * z) | ((uint32_t)tv[z] >> 2) << 12; },
+        [&](int z, int at) { s_l7[at] = (uint16_t)(tid + NC * z); s_r7[at] = tv[z] >> 2; });
     __syncthreads();
     const int n6 = min((int)(s_cnt & 0xFFFF), N6), n7 = min((int)(s_cnt >> 16), N7);
 
@@ -622,7 +621,7 @@ __global__ __launch_bounds__(THREADS) void k_grad_force(GradArgs<T> A, StepConst
         cnv[m] = T(0);
         if (i < n7) {
             const int e = s_l7[i];
-            cnv[m] = A.cnorm[i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / (LX * LY))];
+            cnv[m] = A.cnorm[i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC)];
         }
     }
 
