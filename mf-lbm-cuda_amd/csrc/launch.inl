@@ -86,11 +86,11 @@ void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_
         B.tiles_x = (A.d.nx + 4 + NX - 1) / NX;
         B.tiles_y = (A.d.ny + 4 + NY - 1) / NY;
         const unsigned gn = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + 4 + NZ - 1) / NZ));
-        dev::k_grad_normals<T, NX, NY, NZ, 256><<<gn, 256, 0, s>>>(B, P);
+        dev::k_grad_normals<T, NX, NY, NZ><<<gn, (NX + 4) * (NY + 4), 0, s>>>(B, P);
         B.tiles_x = (A.d.nx + FX - 1) / FX;
         B.tiles_y = (A.d.ny + FY - 1) / FY;
         const unsigned gf = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + FZ - 1) / FZ));
-        dev::k_grad_force<T, FX, FY, FZ, 256><<<gf, 256, 0, s>>>(B, P);
+        dev::k_grad_force<T, FX, FY, FZ><<<gf, (FX + 4) * (FY + 4), 0, s>>>(B, P);
         return;
     }
     const int chunks = (A.d.nz + A.kchunk - 1) / A.kchunk;
