@@ -31,3 +31,14 @@ def test_learned_phase_field_start_32():
         # the learned field is the steady one on most pore nodes
         assert r["predicted_phase_agreement"] > 0.8, r
         assert np.isfinite(r["S_warm"]) and np.isfinite(r["S_plain"])
+        # at this size the learned start lands on the FP64 cold start's steady saturation
+        # (measured 0.3 % and 0.003 %; a few percent is what another drainage path costs)
+        assert abs(r["S_warm"] - r["S_cold"]) < 0.02 * r["S_cold"], r
+    # what the network gains over the method without it (the reference's start in FP32):
+    # fewer steps and less wall time to the steady test, over the held-out packs (measured
+    # 50 000 against 99 000 steps)
+    warm_steps = sum(r["warm_steps"] for r in rows)
+    plain_steps = sum(r["plain_steps"] for r in rows)
+    say(f"held-out steps: learned {warm_steps}, reference start {plain_steps}")
+    assert warm_steps < plain_steps
+    assert sum(r["warm_s"] for r in rows) < sum(r["plain_s"] for r in rows)
