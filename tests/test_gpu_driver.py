@@ -324,3 +324,39 @@ def test_driver_random_initial_distribution(tmp_path):
     assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
     assert np.array_equal(body[2 + 38 * s1:], ref.download("phi").ravel())
     ref.close()
+
+
+def test_driver_checkpoint_and_restart_at_256(tmp_path):
+    """Config 2's 256^3 drainage through the driver at its real size: 40 steps with the full
+    flow-field VTK, then a restart for 20 more; both checkpoints (5.2 GB of populations) equal
+    the Python-driven state bit for bit."""
+    root = str(tmp_path)
+    geo, ctrl = cases.drainage256()
+    nx, ny, nz = int(ctrl["nxGlobal"]), int(ctrl["nyGlobal"]), int(ctrl["nzGlobal"])
+    timers = dict(monitor_timer=20, monitor_profile_timer_ratio=2, ntime_visual=40, ntime_animation=40,
+                  computation_time_timer=20, display_steps_timer=20, n_exclude_inlet=8, n_exclude_outlet=8,
+                  steady_state_option=0, checkpoint_save_timer=1e3, checkpoint_2rd_save_timer=1e3,
+                  simulation_duration_timer=1e3)
+    ctrl = dict(ctrl, max_time_step=39, **timers)
+    write_case(root, ctrl, geo)
+    say("driver: 256^3 new simulation, 40 steps")
+    run_driver(root)
+    vtk = os.path.join(root, "results/out3.field_data/full_flow_field/full_flow_field_0000000040.vtk")
+    assert os.path.getsize(vtk) > 5 * 4 * nx * ny * nz
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    say("reference: python API, 40 steps")
+    ref = reference_run(dict(ctrl), geo, 40)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 41
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    assert np.array_equal(body[2 + 38 * s1:], ref.download("phi").ravel())
+    del body
+    write_case(root, dict(ctrl, max_time_step=19), geo, status="continue_simulation")
+    say("driver: 256^3 continue_simulation, steps 41..60")
+    out = run_driver(root)
+    assert "Simulation ended after\t60 iterations" in out
+    ref.run(41, 20)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 61
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    ref.close()
