@@ -153,7 +153,15 @@ constexpr int GRAD_SHAPE_DEFAULT = 0, GRAD_SHAPE_FP32 = 3;
 // the two-pass chain on 3-D tiles (k_grad_normals, k_grad_force): the shape after the
 // march shapes
 constexpr int GRAD_SHAPE_TILES = GRAD_NSHAPES;
-constexpr int GRAD_TILE_N[3] = {16, 8, 4}, GRAD_TILE_F[3] = {16, 8, 4};
+// owned nodes (x, y, z) of a workgroup of k_grad_normals (N) and k_grad_force (F), per precision
+template <typename T>
+struct GradTiles {
+    static constexpr int N[3] = {16, 8, 4}, F[3] = {16, 8, 4};
+};
+template <>
+struct GradTiles<float> {
+    static constexpr int N[3] = {16, 8, 4}, F[3] = {16, 8, 4};
+};
 // shortest z-run for which the FP64 march is chosen by default
 constexpr int GRAD_MARCH_MIN_RUN = 64;
 

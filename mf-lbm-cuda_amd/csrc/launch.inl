@@ -79,8 +79,8 @@ void Launch<T>::curvature(const Fields<T>& F, const StepConst<T>& P, const NodeL
 template <typename T>
 void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_t s) {
     if (A.shape == GRAD_SHAPE_TILES) {
-        constexpr int NX = GRAD_TILE_N[0], NY = GRAD_TILE_N[1], NZ = GRAD_TILE_N[2];
-        constexpr int FX = GRAD_TILE_F[0], FY = GRAD_TILE_F[1], FZ = GRAD_TILE_F[2];
+        constexpr int NX = GradTiles<T>::N[0], NY = GradTiles<T>::N[1], NZ = GradTiles<T>::N[2];
+        constexpr int FX = GradTiles<T>::F[0], FY = GradTiles<T>::F[1], FZ = GradTiles<T>::F[2];
         GradArgs<T> B = A;
         // normals over [-1, n+2]^3, forces over [1, n]^3
         B.tiles_x = (A.d.nx + 4 + NX - 1) / NX;
