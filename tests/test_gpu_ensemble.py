@@ -162,3 +162,38 @@ def test_init_fields_starts_from_the_given_state():
         err = np.linalg.norm(fb[d, :3] - fa[d, :3]) / np.linalg.norm(fa[d, :3])
         assert err < 1e-4, err
         assert abs(out_b["permeability"][d] / out_a["permeability"][d] - 1) < 1e-4
+
+
+def test_config4_2048_domains_at_real_size():
+    """Config 4 at its real size: 2048 seeded 64^3 sphere packs (porosity U[0.30, 0.50],
+    radius U[4, 10], body force 1e-6) in one ensemble, each to a relative velocity change
+    below 1e-7 per step.  Every domain must become steady with a positive permeability, and
+    three domains spread over the batch, solved again on their own, must give the same step
+    counts, permeabilities and fields bit for bit (the domains do not interact)."""
+    n, count, check, tol = 64, 2048, 100, 1e-5
+    say(f"config 4: {count} geometries of {n}^3")
+    geos, _ = ensemble.sphere_pack_ensemble(range(count), n, workers=16)
+    _, p, _ = warmstart.single_phase_case(0, n)
+    ens = ensemble.Ensemble(p, geos)
+    done = 0
+    while done < 60000:
+        out = ens.solve(check, tol, 2000)
+        done += 2000
+        active = int((out["steps"] < 0).sum())
+        say(f"step {done}: {count - active} of {count} domains steady")
+        if active == 0:
+            break
+    assert (out["steps"] > 0).all()
+    assert np.isfinite(out["permeability"]).all() and (out["permeability"] > 0).all()
+    sample = [0, 1023, 2047]
+    fields = {d: ens.fields(first=d, count=1)[0] for d in sample}
+    ens.close()
+    for d in sample:
+        one = ensemble.Ensemble(p, geos[d:d + 1])
+        o = one.solve(check, tol, 60000)
+        f = one.fields()[0]
+        one.close()
+        say(f"domain {d}: {o['steps'][0]} steps alone, {out['steps'][d]} in the batch")
+        assert o["steps"][0] == out["steps"][d]
+        assert o["permeability"][0] == out["permeability"][d]
+        assert np.array_equal(f, fields[d])
