@@ -413,10 +413,17 @@ __device__ __forceinline__ void with_cells(const Sparse<T>& S, T rho_ref, CellOf
 // conv: the convective-outlet state, 38 planes of populations [g][e] then phi.
 template <typename T, bool AFTER_ODD>
 __global__ __launch_bounds__(256) void k_open_bc(Sparse<T> S, Fields<T> F, StepConst<T> P, const int32_t* __restrict__ zmap,
-                                                 int in_bc, int out_bc, const T* __restrict__ w_in, T* __restrict__ conv) {
+                                                 int in_bc, int out_bc, const T* __restrict__ w_in, T* __restrict__ conv,
+                                                 int split) {
     const Dims& d = F.d;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= d.nx * d.ny) return;
+    int t = blockIdx.x * 256 + threadIdx.x;
+    // split: the inlet columns and the outlet columns on threads of their own (the two
+    // planes share no cell); otherwise one thread does both ends of its column
+    if (t >= (split ? 2 : 1) * d.nx * d.ny) return;
+    if (split) {
+        if (t < d.nx * d.ny) out_bc = 0;
+        else { in_bc = 0; t -= d.nx * d.ny; }
+    }
     const int i = t % d.nx + 1, j = t / d.nx + 1, nz = d.nz;
     const int64_t P2 = (int64_t)d.X1 * d.Y1, plane = 19 * P2;
     auto cell_of = [&](int a, int b, int c, int e) -> int64_t {

@@ -38,9 +38,11 @@ void Launch<T>::periodic_phi(T* phi, const Dims& d, int kper, int jper, hipStrea
 template <typename T>
 void Launch<T>::open_bc(const Sparse<T>& S, const Fields<T>& F, const StepConst<T>& P, const int32_t* zmap, int in_bc,
                         int out_bc, const T* w_in, T* conv, bool after_odd, hipStream_t s) {
-    const int n = F.d.nx * F.d.ny;
-    if (after_odd) dev::k_open_bc<T, true><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv);
-    else dev::k_open_bc<T, false><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv);
+    // inlet and outlet on separate threads when both are set and their planes are apart
+    const int split = in_bc && out_bc && F.d.nz >= 6;
+    const int n = (split ? 2 : 1) * F.d.nx * F.d.ny;
+    if (after_odd) dev::k_open_bc<T, true><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv, split);
+    else dev::k_open_bc<T, false><<<blocks(n), 256, 0, s>>>(S, F, P, zmap, in_bc, out_bc, w_in, conv, split);
 }
 
 template <typename T>
