@@ -70,7 +70,9 @@ struct mflbm_handle {
     double* macro_buf = nullptr;               // mflbm_macro: one s1 field
     bool vprev_valid = false;
     // captured step graphs: [0] even step, [1] odd step
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // step graphs: [0] even and [1] odd single steps, [2] / [3] GRAPH_BLOCK steps starting with
+    // an even / odd one
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     double rho_ref = 1.0;  // FP32 storage reference density (ShiftedStorage)
     std::string err;
 };
@@ -227,7 +229,7 @@ int materialize_gradient(mflbm_handle* h) {
 }
 
 void free_all(mflbm_handle* h) {
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < 4; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
@@ -273,13 +275,19 @@ int field_info(const mflbm_handle* h, int field, int64_t* count, size_t* elem, v
     }
 }
 
+// Steps per block graph: one launch for GRAPH_BLOCK steps spares the host the per-launch cost
+// that dominates small domains (64^3: 0.055 ms per step, 40 us of it kernels).
+constexpr int GRAPH_BLOCK = 8;
+
 int build_graphs(mflbm_handle* h) {
     std::lock_guard<std::recursive_mutex> lock(device_lock());
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < 4; ++g) {
         if (h->graph[g]) continue;
         hipGraph_t graph = nullptr;
         HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        step_dispatch(h, g == 1, nullptr);
+        if (g < 2) step_dispatch(h, g == 1, nullptr);
+        else
+            for (int k = 0; k < GRAPH_BLOCK; ++k) step_dispatch(h, ((g - 2) + k) % 2 == 1, nullptr);
         hipError_t e = hipStreamEndCapture(h->stream, &graph);
         if (e != hipSuccess) return fail(h, "graph capture failed: %s", hipGetErrorString(e));
         e = hipGraphInstantiate(&h->graph[g], graph, nullptr, nullptr, 0);
@@ -291,7 +299,7 @@ int build_graphs(mflbm_handle* h) {
 
 void invalidate_graphs(mflbm_handle* h) {
     std::lock_guard<std::recursive_mutex> lock(device_lock());
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < 4; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
 }
 
@@ -686,7 +694,10 @@ int mflbm_run(mflbm_handle* h, int32_t ntime0, int32_t nsteps) {
     if (!h) return -1;
     if (nsteps <= 0) return 0;
     if (build_graphs(h)) return -1;
-    for (int32_t t = ntime0; t < ntime0 + nsteps; ++t) HIPCHK(h, hipGraphLaunch(h->graph[(t % 2) != 0], h->stream));
+    const int32_t end = ntime0 + nsteps;
+    int32_t t = ntime0;
+    for (; end - t >= GRAPH_BLOCK; t += GRAPH_BLOCK) HIPCHK(h, hipGraphLaunch(h->graph[2 + ((t % 2) != 0)], h->stream));
+    for (; t < end; ++t) HIPCHK(h, hipGraphLaunch(h->graph[(t % 2) != 0], h->stream));
     h->grad_dirty = !h->split_gradient;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
