@@ -327,6 +327,8 @@ def evaluate(ws, seeds, n=64, fine=False, precision=gpu.FP32, tol=1e-4, check=20
 
 def summarise(rows):
     rows = [r for r in rows if not r.get("blew_up")]
+    if not rows:
+        return {"blew_up": True}
     tot = {k: float(sum(r[k] for r in rows)) for k in ("cold_steps", "plain_steps", "warm_steps", "cold_s", "plain_s",
                                                         "warm_s")}
     return {"warm_vs_cold_wall": tot["warm_s"] / tot["cold_s"], "plain_vs_cold_wall": tot["plain_s"] / tot["cold_s"],
@@ -344,6 +346,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--train", type=int, default=32)
     ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--first-eval-seed", type=int, default=1000, help="held-out seeds start here")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--fine", action="store_true", help="evaluate on geometries twice as fine as --n")
@@ -371,7 +374,7 @@ def main(argv=None):
         if args.save:
             ws.save(args.save)
         del data
-    rows = evaluate(ws, range(1000, 1000 + args.eval), args.n, args.fine, tol=args.tol,
+    rows = evaluate(ws, range(args.first_eval_seed, args.first_eval_seed + args.eval), args.n, args.fine, tol=args.tol,
                     check=2 * args.check if args.fine else args.check, log=log)
     res.update(summarise(rows), fine=args.fine, eval=rows)
     text = json.dumps(res)
