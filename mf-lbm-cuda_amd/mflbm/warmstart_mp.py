@@ -269,6 +269,69 @@ def field_agreement(phi_a, phi_b, geo):
     return float(((phi_a > 0) == (phi_b > 0))[pore].mean())
 
 
+def _case_of(s, n, fine):
+    return fine_case(s, n) if fine else drainage_case(s, n)
+
+
+def evaluate_cold(s, n=64, fine=False, tol=1e-4, check=2000, max_steps=400000, log=print):
+    """(row, phase sign (nz, ny, nx) int8) of held-out seed s: the reference's start in FP64,
+    the arbiter, run to the steady saturation."""
+    geo, params, meta = _case_of(s, n, fine)
+    row = dict(meta, n=geo.shape[0])
+    sim = gpu.Solver(params, geo, gpu.FP64)
+    t0 = time.perf_counter()
+    row["cold_steps"], row["S_cold"] = run_to_steady_saturation(sim, params, tol=tol, check=check,
+                                                                max_steps=max_steps, log=log)
+    row["cold_s"] = time.perf_counter() - t0
+    row["fl1_cold"] = fluid1_flux(sim)
+    sign = np.sign(sim.download("phi")[4:-4, 4:-4, 4:-4]).astype(np.int8)
+    sim.close()
+    log(f"held-out {s} ({geo.shape[0]}^3): FP64 reference start {row['cold_steps']} steps "
+        f"{row['cold_s']:.1f} s, S {row['S_cold']:.5f}")
+    if not np.isfinite(row["S_cold"]):
+        row["blew_up"] = True
+    return row, sign
+
+
+def evaluate_rest(ws, row, cold_sign, fine=False, precision=gpu.FP32, tol=1e-4, check=2000, max_steps=400000,
+                  fallback_step=500, fallback_residual=1e-4, log=print):
+    """Completes a row of evaluate_cold: the same start in `precision` (the method without
+    the network) and the learned start in `precision` with the fallback rule."""
+    s = row["seed"]
+    geo, params, _ = _case_of(s, row["n"] // 2 if fine else row["n"], fine)
+    kw = dict(tol=tol, check=check, max_steps=max_steps)
+    sim = gpu.Solver(params, geo, precision)
+    t0 = time.perf_counter()
+    row["plain_steps"], row["S_plain"] = run_to_steady_saturation(sim, params, **kw)
+    row["plain_s"] = time.perf_counter() - t0
+    row["fl1_plain"] = fluid1_flux(sim)
+    t0 = time.perf_counter()
+    phi0 = ws.predict_fine(geo) if fine else ws.predict(geo)
+    row["predict_s"] = time.perf_counter() - t0
+    row["predicted_phase_agreement"] = field_agreement(phi0, cold_sign, geo)
+    info = {}
+    steps, sat = run_to_steady_saturation(sim, params, phi0, probe=(fallback_step, fallback_residual), info=info,
+                                          **kw)
+    row["warm_residual_at_probe"] = info.get("residual")
+    row["fallback"] = sat is None
+    if sat is None:
+        lost = steps
+        steps, sat = run_to_steady_saturation(sim, params, **kw)
+        steps += lost
+    row["warm_s"] = time.perf_counter() - t0
+    row["warm_steps"], row["S_warm"] = steps, sat
+    row["fl1_warm"] = fluid1_flux(sim)
+    sim.close()
+    for k in ("plain", "warm"):
+        row[f"S_{k}_rel_diff"] = abs(row[f"S_{k}"] - row["S_cold"]) / row["S_cold"]
+        row[f"fl1_{k}_rel_diff"] = abs(row[f"fl1_{k}"] - row["fl1_cold"]) / abs(row["fl1_cold"])
+    log(f"held-out {s}: FP32 reference start {row['plain_steps']} steps {row['plain_s']:.1f} s, "
+        f"S {row['S_plain']:.5f}; learned start {row['warm_steps']} steps {row['warm_s']:.1f} s, "
+        f"S {row['S_warm']:.5f} (fallback {row['fallback']}, phases {row['predicted_phase_agreement']:.3f} "
+        f"as the FP64 steady state)")
+    return row
+
+
 def evaluate(ws, seeds, n=64, fine=False, precision=gpu.FP32, tol=1e-4, check=2000, max_steps=400000,
              fallback_step=500, fallback_residual=1e-4, log=print):
     """Per held-out geometry: the reference's start in FP64 (the arbiter), the same start in
@@ -276,51 +339,9 @@ def evaluate(ws, seeds, n=64, fine=False, precision=gpu.FP32, tol=1e-4, check=20
     the fallback rule, each run to the steady saturation."""
     rows = []
     for s in seeds:
-        geo, params, meta = fine_case(s, n) if fine else drainage_case(s, n)
-        row = dict(meta, n=geo.shape[0])
-        kw = dict(tol=tol, check=check, max_steps=max_steps)
-        sim = gpu.Solver(params, geo, gpu.FP64)
-        t0 = time.perf_counter()
-        row["cold_steps"], row["S_cold"] = run_to_steady_saturation(sim, params, **kw)
-        row["cold_s"] = time.perf_counter() - t0
-        row["fl1_cold"] = fluid1_flux(sim)
-        phi_cold = sim.download("phi")[4:-4, 4:-4, 4:-4]
-        sim.close()
-        log(f"held-out {s} ({geo.shape[0]}^3): FP64 reference start {row['cold_steps']} steps "
-            f"{row['cold_s']:.1f} s, S {row['S_cold']:.5f}")
-        if not np.isfinite(row["S_cold"]):
-            row["blew_up"] = True
-            rows.append(row)
-            continue
-        sim = gpu.Solver(params, geo, precision)
-        t0 = time.perf_counter()
-        row["plain_steps"], row["S_plain"] = run_to_steady_saturation(sim, params, **kw)
-        row["plain_s"] = time.perf_counter() - t0
-        row["fl1_plain"] = fluid1_flux(sim)
-        t0 = time.perf_counter()
-        phi0 = ws.predict_fine(geo) if fine else ws.predict(geo)
-        row["predict_s"] = time.perf_counter() - t0
-        row["predicted_phase_agreement"] = field_agreement(phi0, phi_cold, geo)
-        info = {}
-        steps, sat = run_to_steady_saturation(sim, params, phi0, probe=(fallback_step, fallback_residual), info=info,
-                                              **kw)
-        row["warm_residual_at_probe"] = info.get("residual")
-        row["fallback"] = sat is None
-        if sat is None:
-            lost = steps
-            steps, sat = run_to_steady_saturation(sim, params, **kw)
-            steps += lost
-        row["warm_s"] = time.perf_counter() - t0
-        row["warm_steps"], row["S_warm"] = steps, sat
-        row["fl1_warm"] = fluid1_flux(sim)
-        sim.close()
-        for k in ("plain", "warm"):
-            row[f"S_{k}_rel_diff"] = abs(row[f"S_{k}"] - row["S_cold"]) / row["S_cold"]
-            row[f"fl1_{k}_rel_diff"] = abs(row[f"fl1_{k}"] - row["fl1_cold"]) / abs(row["fl1_cold"])
-        log(f"held-out {s}: FP32 reference start {row['plain_steps']} steps {row['plain_s']:.1f} s, "
-            f"S {row['S_plain']:.5f}; learned start {row['warm_steps']} steps {row['warm_s']:.1f} s, "
-            f"S {row['S_warm']:.5f} (fallback {row['fallback']}, phases {row['predicted_phase_agreement']:.3f} "
-            f"as the FP64 steady state)")
+        row, sign = evaluate_cold(s, n, fine, tol, check, max_steps, log)
+        if not row.get("blew_up"):
+            evaluate_rest(ws, row, sign, fine, precision, tol, check, max_steps, fallback_step, fallback_residual, log)
         rows.append(row)
     return rows
 
@@ -355,9 +376,26 @@ def main(argv=None):
     ap.add_argument("--save", default=None)
     ap.add_argument("--load", default=None)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--max-steps", type=int, default=400000)
+    ap.add_argument("--stage", choices=["all", "cold", "rest"], default="all",
+                    help="cold: only the FP64 reference starts, saved to --out and <out>.phase<seed>.npz; "
+                         "rest: the other starts from those files (--cold), so that a large evaluation "
+                         "can run as two jobs")
+    ap.add_argument("--cold", default=None, help="--stage rest: the --out file of the cold stage")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     res = {"n": args.n, "physics": PHYSICS, "tol": args.tol, "check": args.check}
+    check = 2 * args.check if args.fine else args.check
+    seeds = range(args.first_eval_seed, args.first_eval_seed + args.eval)
+    if args.stage == "cold":
+        rows = []
+        for s in seeds:
+            row, sign = evaluate_cold(s, args.n, args.fine, args.tol, check, args.max_steps, log)
+            np.savez_compressed(f"{args.out}.phase{s}.npz", sign=sign)
+            rows.append(row)
+        res.update(stage="cold", eval=rows)
+        _write(res, args.out)
+        return
     if args.load:
         ws = PhaseWarmStart.load(args.load)
     else:
@@ -374,12 +412,24 @@ def main(argv=None):
         if args.save:
             ws.save(args.save)
         del data
-    rows = evaluate(ws, range(args.first_eval_seed, args.first_eval_seed + args.eval), args.n, args.fine, tol=args.tol,
-                    check=2 * args.check if args.fine else args.check, log=log)
+    if args.stage == "rest":
+        with open(args.cold) as fh:
+            rows = json.loads(fh.read())["eval"]
+        for row in rows:
+            if not row.get("blew_up"):
+                with np.load(f"{args.cold}.phase{row['seed']}.npz") as z:
+                    sign = z["sign"]
+                evaluate_rest(ws, row, sign, args.fine, tol=args.tol, check=check, max_steps=args.max_steps, log=log)
+    else:
+        rows = evaluate(ws, seeds, args.n, args.fine, tol=args.tol, check=check, max_steps=args.max_steps, log=log)
     res.update(summarise(rows), fine=args.fine, eval=rows)
+    _write(res, args.out)
+
+
+def _write(res, path):
     text = json.dumps(res)
-    if args.out:
-        with open(args.out, "w") as fh:
+    if path:
+        with open(path, "w") as fh:
             fh.write(text + "\n")
     print(text, flush=True)
 
