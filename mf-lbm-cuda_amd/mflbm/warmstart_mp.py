@@ -378,7 +378,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--stage", choices=["all", "cold", "rest"], default="all",
-                    help="cold: only the FP64 reference starts, saved to --out and <out>.phase<seed>.npz; "
+                    help="cold: only the FP64 reference starts, saved to --out and <out>.phase<seed>; "
                          "rest: the other starts from those files (--cold), so that a large evaluation "
                          "can run as two jobs")
     ap.add_argument("--cold", default=None, help="--stage rest: the --out file of the cold stage")
@@ -391,7 +391,8 @@ def main(argv=None):
         rows = []
         for s in seeds:
             row, sign = evaluate_cold(s, args.n, args.fine, args.tol, check, args.max_steps, log)
-            np.savez_compressed(f"{args.out}.phase{s}.npz", sign=sign)
+            with open(f"{args.out}.phase{s}", "wb") as fh:  # a file object: no suffix appended
+                np.savez_compressed(fh, sign=sign)
             rows.append(row)
         res.update(stage="cold", eval=rows)
         _write(res, args.out)
@@ -417,7 +418,7 @@ def main(argv=None):
             rows = json.loads(fh.read())["eval"]
         for row in rows:
             if not row.get("blew_up"):
-                with np.load(f"{args.cold}.phase{row['seed']}.npz") as z:
+                with np.load(f"{args.cold}.phase{row['seed']}") as z:
                     sign = z["sign"]
                 evaluate_rest(ws, row, sign, args.fine, tol=args.tol, check=check, max_steps=args.max_steps, log=log)
     else:
