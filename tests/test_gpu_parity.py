@@ -85,6 +85,10 @@ CASES = {
     # porous plate (src/main_iteration_GPU.cu:1744-1882) across the sphere pack / in the inlet buffer
     "plate_blocks_fluid1": lambda: small_drainage(n=(20, 22, 28), seed=6, jper=1, plate=1, plate_z=14),
     "plate_blocks_fluid2": lambda: small_drainage(n=(18, 20, 26), seed=4, jper=0, plate=2, plate_z=2),
+    # sizes that no workgroup shape divides, and an axis so short that the inlet and outlet
+    # columns share their threads
+    "awkward_sizes": lambda: small_drainage(n=(13, 11, 9), seed=4, radius=2.5, porosity=0.5, jper=1, buffer=2),
+    "five_planes": lambda: small_drainage(n=(12, 10, 5), seed=2, radius=2.0, porosity=0.6, jper=1, buffer=1),
 }
 
 
