@@ -110,8 +110,8 @@ def kernel_group(name):
         return "probe_blocked"
     if PROBE_KERNEL in name:
         return "probe"
-    for key, short in (("k_collide<double, true>", "collide_odd"), ("k_collide<float, true>", "collide_odd"),
-                       ("k_collide<double, false>", "collide_even"), ("k_collide<float, false>", "collide_even"),
+    for key, short in (("k_collide<double, true", "collide_odd"), ("k_collide<float, true", "collide_odd"),
+                       ("k_collide<double, false", "collide_even"), ("k_collide<float, false", "collide_even"),
                        ("k_gradient<", "gradient"), ("k_grad_normals<", "grad_normals"),
                        ("k_grad_force<", "grad_force")):
         if key in name:

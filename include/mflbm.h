@@ -46,6 +46,10 @@ typedef struct mflbm_params {
     int32_t z_porous_plate;      /* Z_porous_plate, 1 .. nz */
     double uin_avg;              /* mean inlet velocity (velocity inlet, convective outlet):
                                   * capillary_number * surface_tension / fluid1_viscosity */
+    int32_t mrt;                 /* collision model, the reference's compile-time `mrt`
+                                  * (includes/preprocessor.h:3-4; src/main_iteration_GPU.cu:157-187):
+                                  * 1 MRT bounce-back optimised, 2 MRT original (its default),
+                                  * 3 SRT, 4 MRT advection optimised; 0 means 2 */
 } mflbm_params;
 
 /* Run the colour-gradient chain as four kernels per step, one per reference kernel

@@ -117,6 +117,7 @@ StepConst<T> consts(const mflbm_handle* h) {
     c.rho_in = (T)p.rho_in; c.rho_out = (T)p.rho_out; c.relaxation = (T)p.relaxation;
     c.rho_ref = (T)h->rho_ref;
     c.uin_avg = (T)p.uin_avg;
+    c.mrt = p.mrt ? p.mrt : 2;
     return c;
 }
 
@@ -384,6 +385,8 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         h->out_bc = p.outlet_bc;
     }
     if (p.porous_plate < 0 || p.porous_plate > 2) return fail(h, "porous_plate must be 0, 1 or 2");
+    if (p.mrt < 0 || p.mrt > 4) return fail(h, "mrt must be 1 (MRT bounce-back optimised), 2 (MRT original), 3 (SRT) "
+                                               "or 4 (MRT advection optimised), or 0 for 2");
     // With y periodic and z closed the reference launches the plate kernel on the grid left by the
     // periodic-y kernel, which covers only the first rows of the plane (src/main_iteration_GPU.cu:1911-1957).
     if (p.porous_plate && !h->open_z && p.jper)

@@ -41,6 +41,7 @@ struct EnsDev {
     const int32_t* dom_count;  // fluid nodes of each domain
     const int32_t* active;     // 1 while the domain is being stepped
     float nui, force;
+    int mrt;                   // collision model (mflbm_params::mrt)
 };
 
 // AA step of one fluid node (the access pattern of k_collide, one colour).
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(EB) void k_ens_collide(EnsDev E) {
     // harmonic viscosity of collide_node at phi = 1
     const float omega = 1.0f / (6.0f / ((1.0f + 1.0f) * E.nui + (1.0f - 1.0f) * E.nui) + 0.5f);
     float f[19];
-    dev::mrt_collide<float>(ft, m_rho, 0.0f, 0.0f, E.force, omega, f);
+    dev::mrt_collide<float>(ft, m_rho, 0.0f, 0.0f, E.force, omega, f, E.mrt);
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
         const int64_t dst = ODD ? nb[q] : t;
@@ -193,6 +194,7 @@ struct mflbm_ensemble {
     int nd = 0;
     int64_t nx = 0, ny = 0, nz = 0, npt = 0;
     double nu = 0.0, force = 0.0;
+    int mrt = 2;
     int64_t nf = 0, ns = 0, nf_real = 0;
     std::vector<int32_t> start, count;
     float* f = nullptr;
@@ -235,6 +237,7 @@ EnsDev dev_view(const mflbm_ensemble* e) {
     E.dom_start = e->dom_start; E.dom_count = e->dom_count; E.active = e->active;
     E.nui = (float)(1.0 / e->nu);
     E.force = (float)e->force;
+    E.mrt = e->mrt;
     return E;
 }
 
@@ -297,6 +300,7 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
     e->npt = e->nx * e->ny * e->nz;
     e->nu = 1.0 / params->la_nui1;
     e->force = params->force_z;
+    e->mrt = params->mrt ? params->mrt : 2;
     const int64_t npt = e->npt, nx = e->nx, ny = e->ny, nz = e->nz;
     const int nd = n_domains;
 

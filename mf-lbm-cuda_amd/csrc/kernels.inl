@@ -56,9 +56,23 @@ __device__ __forceinline__ T wq(int q) { return q == 0 ? W<T>::w0 : (q <= 6 ? W<
 // 164-345): moments, relaxation towards the equilibrium of density m_rho and velocity
 // u = j + F/2, back-transform into f.  With ShiftedStorage ft are deviations from the rest
 // state and m_rho the density deviation (see collide_node).
+// The relaxation rates follow the collision model the reference selects with its `mrt`
+// macro (src/main_iteration_GPU.cu:157-187): 1 bounce-back optimised, 2 original (the
+// default), 3 SRT, 4 advection optimised; the model is uniform over a launch.
 template <typename T>
-__device__ __forceinline__ void mrt_collide(const T (&ft)[19], T m_rho, T fx, T fy, T fz, T omega, T (&f)[19]) {
-    const T s_nu = omega, s_e = T(1.19), s_e2 = T(1.4), s_pi = T(1.4), s_q = T(1.2), s_t = T(1.98);
+__device__ __forceinline__ void mrt_collide(const T (&ft)[19], T m_rho, T fx, T fy, T fz, T omega, T (&f)[19],
+                                            int mrt = 2) {
+    T s_e = T(1.19), s_e2 = T(1.4), s_pi = T(1.4), s_q = T(1.2), s_t = T(1.98);
+    if (mrt == 1) {
+        s_e = s_e2 = s_pi = omega;
+        s_q = s_t = T(8.0) * (T(2.0) - omega) / (T(8.0) - omega);
+    } else if (mrt == 3) {
+        s_e = s_e2 = s_pi = s_q = s_t = omega;
+    } else if (mrt == 4) {
+        s_e = s_e2 = s_pi = s_t = omega;
+        s_q = (T(6.0) - T(3.0) * omega) / (T(3.0) - omega);
+    }
+    const T s_nu = omega;
 
     const T ux = ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] + T(0.5) * fx;
     const T uy = ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] + T(0.5) * fy;
@@ -165,7 +179,10 @@ __device__ __forceinline__ void mrt_collide(const T (&ft)[19], T m_rho, T fx, T 
 // rest state at rho_ref; the moments are then those of the deviation, in which only
 // m_rho (rho_ref) and m_e (-11 rho_ref) of the rest state differ from zero, so the same
 // relaxation applies with the density deviation in place of the density.
-template <typename T>
+// ORIG: the reference's default model (mrt 2), whose constant rates the compiler folds; the
+// other models read theirs from P.mrt in a kernel of their own, so that the default keeps
+// its registers (the even collide needs 61 VGPRs in FP32 with constants, 118 without).
+template <typename T, bool ORIG = true>
 __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[19], const T (&g2)[19], T cnx, T cny,
                                           T cnz, T ftmp, T (&o1)[19], T (&o2)[19]) {
     constexpr bool SH = ShiftedStorage<T>::value;
@@ -211,7 +228,7 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 
     const T omega = T(1.0) / (T(6.0) / ((T(1.0) + phi) * P.nui1 + (T(1.0) - phi) * P.nui2) + T(0.5));
     T f[19];
-    mrt_collide<T>(ft, m_rho, fx, fy, fz, omega, f);
+    mrt_collide<T>(ft, m_rho, fx, fy, fz, omega, f, ORIG ? 2 : P.mrt);
 
     const T tmp1 = rho1 / den;
     o1[0] = tmp1 * (SH ? f[0] + P.rho_ref * T(rest_weight(0)) : f[0]);
@@ -251,7 +268,7 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 #ifndef MFLBM_COLLIDE_THREADS
 #define MFLBM_COLLIDE_THREADS 256  // threads per collide workgroup (tuning: 128, 256, 512)
 #endif
-template <typename T, bool ODD>
+template <typename T, bool ODD, bool ORIG = true>
 __global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, T* __restrict__ phi, Dims d,
                                                                    StepConst<T> P) {
     const int t = xcd_block() * MFLBM_COLLIDE_THREADS + threadIdx.x;
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, 
         for (int m = 0; m < 4; ++m) rec[m] = a[m];
         packed = S.list[t];
     }
-    const T ph = collide_node<T>(P, g1, g2, rec[0], rec[1], rec[2], rec[3], o1, o2);
+    const T ph = collide_node<T, ORIG>(P, g1, g2, rec[0], rec[1], rec[2], rec[3], o1, o2);
     int i, j, k;
     unpack_ijk(packed, i, j, k);
     if (ODD) __builtin_nontemporal_store(ph, phi + i4(d, i, j, k));

@@ -74,6 +74,7 @@ struct StepConst {
     T gamma, force_z, nui1, nui2, beta, phi_inlet, sa_inject, cos_theta, rho_in, rho_out, relaxation;
     T rho_ref;  // ShiftedStorage: density of the rest state the deviations are taken from
     T uin_avg;  // convective-outlet velocity
+    int mrt;    // collision model 1-4 (mflbm_params::mrt)
 };
 
 struct NodeLists {

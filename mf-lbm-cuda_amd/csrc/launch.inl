@@ -20,8 +20,13 @@ template <typename T>
 void Launch<T>::collide(const Sparse<T>& S, T* phi, const Dims& d, const StepConst<T>& P, bool odd, hipStream_t s) {
     if (S.nf <= 0) return;
     constexpr int b = MFLBM_COLLIDE_THREADS;
-    if (odd) dev::k_collide<T, true><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
-    else dev::k_collide<T, false><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+    if (P.mrt == 2) {
+        if (odd) dev::k_collide<T, true><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+        else dev::k_collide<T, false><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+    } else {  // collision models 1, 3, 4
+        if (odd) dev::k_collide<T, true, false><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+        else dev::k_collide<T, false, false><<<blocks(S.nf, b), b, 0, s>>>(S, phi, d, P);
+    }
 }
 
 template <typename T>

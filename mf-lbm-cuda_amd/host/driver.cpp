@@ -19,7 +19,9 @@
 //     "float" but writes T_P when output_fieldData_precision_cmd is 0);
 //   * the device state is copied to the host only for the outputs that need it;
 //   * precision is a run-time option (--precision fp64|fp32) rather than a build switch;
-//     a checkpoint of either precision can be resumed by either.
+//     a checkpoint of either precision can be resumed by either;
+//   * so is the collision model (--mrt 1-4, the reference's `mrt` macro of
+//     includes/preprocessor.h, default 2).
 //   * the convective-outlet tail of the checkpoint is written when that outlet is active
 //     (outlet_BC = 1 on an open z axis); the reference also writes it, unused, when
 //     outlet_BC = 1 on a closed or periodic z axis.  Files with or without it load.
@@ -219,8 +221,9 @@ void check_control(const Control& c) {
 // ---------------------------------------------------------------------------------------
 constexpr double PI = 3.14159265358979323846;
 
-mflbm_params derive(const Control& c, int precision) {
+mflbm_params derive(const Control& c, int precision, int mrt) {
     mflbm_params p{};
+    p.mrt = mrt;
     p.nx = c.nx; p.ny = c.ny; p.nz = c.nz;
     p.iper = c.iper; p.jper = c.jper; p.kper = c.kper;
     p.wall_x_min = c.wall[0]; p.wall_x_max = c.wall[1];
@@ -367,7 +370,7 @@ void put_be(std::ofstream& out, T v) {
 // ---------------------------------------------------------------------------------------
 class Simulation {
 public:
-    Simulation(const Control& c, int precision) : c_(c), prec_(precision) {
+    Simulation(const Control& c, int precision, int mrt) : c_(c), prec_(precision), mrt_(mrt) {
         nx_ = c.nx; ny_ = c.ny; nz_ = c.nz;
         s1_ = (nx_ + 2) * (ny_ + 2) * (nz_ + 2);
         s2_ = (nx_ + 4) * (ny_ + 4) * (nz_ + 4);
@@ -445,6 +448,7 @@ private:
 
     Control c_;  // a copy: the velocity inlet rescales the timers by the injected volume
     const int prec_;
+    const int mrt_;  // collision model (--mrt)
     mflbm_handle* h_ = nullptr;
     mflbm_params p_{};
     int64_t nx_, ny_, nz_, s1_, s2_, s4_;
@@ -907,7 +911,7 @@ int Simulation::run(const std::string& job_status) {
                           "results/out3.field_data/phase_distribution/", "results/out3.field_data/full_flow_field/"})
         make_dir(d);
     const std::vector<int8_t> geo = read_geometry(c_);
-    p_ = derive(c_, prec_);
+    p_ = derive(c_, prec_, mrt_);
     std::vector<double> ck_pdf, ck_phi, ck_conv;
     if (job_status == "continue_simulation") {
         std::cout << "loading checkpoint data ... ";
@@ -1056,9 +1060,11 @@ int Simulation::run(const std::string& job_status) {
 }
 
 void usage() {
-    std::cout << "usage: mflbm_run [--precision fp64|fp32] [--check]\n"
+    std::cout << "usage: mflbm_run [--precision fp64|fp32] [--mrt 1|2|3|4] [--check]\n"
                  "Runs the simulation described by input/job_status.txt, input/simulation_control.txt\n"
                  "and input/Geometry_File_Path.txt in the current directory; outputs go to results/.\n"
+                 "--mrt selects the collision model, the reference's compile-time mrt switch: 1 MRT\n"
+                 "bounce-back optimised, 2 MRT original (default), 3 SRT, 4 MRT advection optimised.\n"
                  "--check reads and validates the input files without touching the GPU.\n";
 }
 
@@ -1066,6 +1072,7 @@ void usage() {
 
 int main(int argc, char** argv) {
     int precision = MFLBM_FP64;
+    int mrt = 2;
     bool check_only = false;
     for (int a = 1; a < argc; ++a) {
         const std::string arg = argv[a];
@@ -1074,6 +1081,10 @@ int main(int argc, char** argv) {
             if (v == "fp64") precision = MFLBM_FP64;
             else if (v == "fp32") precision = MFLBM_FP32;
             else { usage(); return 2; }
+        } else if (arg == "--mrt" && a + 1 < argc) {
+            const std::string v = argv[++a];
+            if (v.size() != 1 || v[0] < '1' || v[0] > '4') { usage(); return 2; }
+            mrt = v[0] - '0';
         } else if (arg == "--check") {
             check_only = true;
         } else if (arg == "--help" || arg == "-h") {
@@ -1098,7 +1109,7 @@ int main(int argc, char** argv) {
             if (c.external_geometry == 1) (void)read_geometry(c);
             return 0;
         }
-        Simulation sim(c, precision);
+        Simulation sim(c, precision, mrt);
         return sim.run(status);
     } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;

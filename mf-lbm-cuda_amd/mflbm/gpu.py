@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("rho_in", C.c_double), ("rho_out", C.c_double), ("relaxation", C.c_double),
         ("precision", C.c_int32), ("flags", C.c_int32),
         ("porous_plate", C.c_int32), ("z_porous_plate", C.c_int32), ("uin_avg", C.c_double),
+        ("mrt", C.c_int32),
     ]
 
 
@@ -132,6 +133,8 @@ def make_params(p: dict, precision=FP64, flags=0) -> Params:
             cp.precision = precision
         elif name == "flags":
             cp.flags = flags
+        elif name == "mrt":
+            cp.mrt = int(p.get("mrt", 2))
         else:
             setattr(cp, name, p[name])
     return cp

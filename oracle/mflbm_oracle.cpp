@@ -14,7 +14,9 @@
 //  sums below are spelled out term by term.
 //
 //  Pinning (how the restatement itself is checked, see tests/test_oracle_*.py):
-//    collide_node   : independent matrix MRT + Guo forcing + LKR recolouring (numpy)
+//    collide_node   : independent matrix MRT + Guo forcing + LKR recolouring (numpy), for
+//                     each of the four collision models; SRT at rest == BGK towards the
+//                     collision's rest weights (4/21, 13/126, 1/63)
 //    AA streaming   : two steps with zero-force rest populations vs explicit pull stream
 //    gradient chain : ISO4 exact on linear/quadratic fields, sphere curvature ~ 2/R,
 //                     wetting angle n.s_n = cos(theta') after the secant iteration
@@ -62,6 +64,7 @@ typedef struct oracle_params {
     int32_t porous_plate;    // porous_plate_cmd: 0 none, 1 blocks fluid 1, 2 blocks fluid 2
     int32_t z_porous_plate;  // Z_porous_plate
     double uin_avg;          // capillary_number * surface_tension / nu1 (src/Init_multiphase.cpp:262)
+    int32_t mrt;             // collision model (includes/preprocessor.h:3-4), 0 = 2
 } oracle_params;
 
 typedef struct oracle_monitor {
@@ -432,6 +435,24 @@ void init_pdf(OracleSim& S, const double* rho, const double* u, const double* v,
 //  out1, out2: outgoing post-collision populations by direction.
 //  Returns phi of the node.
 // =====================================================================================
+// Relaxation rates of the collision models the reference compiles, selected by its `mrt`
+// macro (src/main_iteration_GPU.cu:157-187): 1 bounce-back optimised, 2 original (the
+// default), 3 SRT, 4 advection optimised.  s_nu = omega in all four.
+struct MrtRates {
+    double e, e2, pi, q, t;
+};
+inline MrtRates mrt_rates(int mrt, double omega) {
+    switch (mrt) {
+        case 1: {
+            const double q = 8.0 * (2.0 - omega) / (8.0 - omega);
+            return {omega, omega, omega, q, q};
+        }
+        case 3: return {omega, omega, omega, omega, omega};
+        case 4: return {omega, omega, omega, (6.0 - 3.0 * omega) / (3.0 - omega), omega};
+        default: return {1.19, 1.4, 1.4, 1.2, 1.98};
+    }
+}
+
 inline double collide_node(const oracle_params& P, const double* g1, const double* g2,
                            double cnx, double cny, double cnz, double curv, double cnorm,
                            double* out1, double* out2) {
@@ -445,7 +466,8 @@ inline double collide_node(const oracle_params& P, const double* g1, const doubl
     const double fx = tmp * cnx, fy = tmp * cny, fz = tmp * cnz + P.force_z;
 
     const double omega = 1.0 / (6.0 / ((1.0 + phi) * P.la_nui1 + (1.0 - phi) * P.la_nui2) + 0.5);
-    const double s_nu = omega, s_e = 1.19, s_e2 = 1.4, s_pi = 1.4, s_q = 1.2, s_t = 1.98;  // mrt == 2
+    const MrtRates R = mrt_rates(P.mrt, omega);
+    const double s_nu = omega, s_e = R.e, s_e2 = R.e2, s_pi = R.pi, s_q = R.q, s_t = R.t;
 
     const double den = rho1 + rho2;
     const double ux = ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] + 0.5 * fx;

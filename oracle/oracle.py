@@ -27,6 +27,7 @@ class OracleParams(C.Structure):
         ("lbm_beta", C.c_double), ("phi_inlet", C.c_double), ("sa_inject", C.c_double), ("cos_theta", C.c_double),
         ("rho_in", C.c_double), ("rho_out", C.c_double), ("relaxation", C.c_double),
         ("porous_plate", C.c_int32), ("z_porous_plate", C.c_int32), ("uin_avg", C.c_double),
+        ("mrt", C.c_int32),
     ]
 
 
@@ -84,7 +85,7 @@ def lib():
 def make_params(p: dict) -> OracleParams:
     op = OracleParams()
     for name, _ in OracleParams._fields_:
-        setattr(op, name, p[name])
+        setattr(op, name, p.get(name, 2) if name == "mrt" else p[name])
     return op
 
 

@@ -43,7 +43,20 @@ M = moment_matrix()
 MINV = np.linalg.inv(M)
 
 
-def collide(g1, g2, n, curv, cnorm, gamma, force_z, nui1, nui2, beta):
+def model_rates(model, omega):
+    """(s_e, s_e2, s_pi, s_q, s_t) of the reference's four collision models (its `mrt` macro):
+    1 bounce-back optimised, 2 original, 3 single relaxation time, 4 advection optimised."""
+    if model == 1:
+        sq = 8 * (2 - omega) / (8 - omega)
+        return omega, omega, omega, sq, sq
+    if model == 3:
+        return omega, omega, omega, omega, omega
+    if model == 4:
+        return omega, omega, omega, (6 - 3 * omega) / (3 - omega), omega
+    return 1.19, 1.4, 1.4, 1.2, 1.98
+
+
+def collide(g1, g2, n, curv, cnorm, gamma, force_z, nui1, nui2, beta, model=2):
     """Post-collision (o1, o2) and phi for populations g1, g2 of shape (19, ...)."""
     g1 = np.asarray(g1, dtype=np.float64)
     g2 = np.asarray(g2, dtype=np.float64)
@@ -57,8 +70,10 @@ def collide(g1, g2, n, curv, cnorm, gamma, force_z, nui1, nui2, beta):
     j = np.tensordot(E.T, f, axes=(1, 0)) + 0.5 * F  # "velocity" of the incompressible form
     u2 = (j ** 2).sum(0)
     omega = 1.0 / (6.0 / ((1 + phi) * nui1 + (1 - phi) * nui2) + 0.5)
-    s = np.array([0, 1.19, 1.4, 0, 1.2, 0, 1.2, 0, 1.2, 1, 1.4, 1, 1.4, 1, 1, 1, 1.98, 1.98, 1.98], dtype=np.float64)
-    S = np.broadcast_to(s.reshape((19,) + (1,) * (f.ndim - 1)), (19,) + f.shape[1:]).copy()
+    S = np.zeros((19,) + f.shape[1:])
+    se, se2, spi, sq, st = model_rates(model, omega)
+    for r, v in ((1, se), (2, se2), (4, sq), (6, sq), (8, sq), (10, spi), (12, spi), (16, st), (17, st), (18, st)):
+        S[r] = v
     for r in (9, 11, 13, 14, 15):
         S[r] = omega
     m = np.tensordot(M, f, axes=(1, 0))

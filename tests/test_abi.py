@@ -23,9 +23,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header_layout():
-    # mflbm_params: 3 int64 + 11 int32 (+4 pad) + 11 double + 4 int32 + 1 double
-    assert ctypes.sizeof(gpu.Params) == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 4 * 4 + 8
+    # mflbm_params: 3 int64 + 11 int32 (+4 pad) + 11 double + 4 int32 + 1 double + 1 int32 (+4 pad)
+    assert ctypes.sizeof(gpu.Params) == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 4 * 4 + 8 + 4 + 4
     assert gpu.Params.uin_avg.offset == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 4 * 4
+    assert gpu.Params.mrt.offset == 3 * 8 + 11 * 4 + 4 + 11 * 8 + 4 * 4 + 8
     assert ctypes.sizeof(gpu.MonitorOut) == 7 * 8 + 3 * 8 + 8
 
 

@@ -73,3 +73,16 @@ def test_geometry_larger_than_domain_is_an_error(tmp_path):
     write_case(str(tmp_path), ctrl, np.zeros((30, 16, 16), np.int8))
     r = check(tmp_path)
     assert r.returncode == 1 and "Domain size is smaller" in r.stderr
+
+
+@pytest.mark.parametrize("model, rc", [("1", 0), ("3", 0), ("4", 0), ("0", 2), ("5", 2), ("x", 2)])
+def test_collision_model_option(tmp_path, model, rc):
+    """--mrt takes the reference's compile-time collision models 1-4; anything else is a
+    usage error before any input is read."""
+    ctrl, geo = small_case()
+    write_case(str(tmp_path), ctrl, geo)
+    r = subprocess.run([DRIVER, "--mrt", model, "--check"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == rc, r.stdout + r.stderr
+    if rc:
+        assert "usage: mflbm_run" in r.stdout

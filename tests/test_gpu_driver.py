@@ -360,3 +360,26 @@ def test_driver_checkpoint_and_restart_at_256(tmp_path):
     assert nt == 61
     assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
     ref.close()
+
+
+def test_driver_collision_model_option(tmp_path):
+    """--mrt 3 (the reference's SRT build) runs the SRT collision: the checkpoint equals the
+    API-driven state with mrt = 3 bit for bit, and differs from the default model's."""
+    root = str(tmp_path)
+    ctrl, geo = case(49)  # steps 1..50
+    write_case(root, ctrl, geo)
+    say("driver: --mrt 3, 50 steps")
+    run_driver(root, "--mrt", "3")
+    nx, ny, nz = N
+    s1 = (nx + 2) * (ny + 2) * (nz + 2)
+    nt, body = read_checkpoint(os.path.join(root, "results/out2.checkpoint/id0000"))
+    assert nt == 51
+    p = config.derive(ctrl)
+    ref = gpu.Solver(dict(p, mrt=3), geo, gpu.FP64)
+    ref.setup()
+    ref.run(1, 50)
+    assert np.array_equal(body[2:2 + 38 * s1], ref.download("pdf").ravel())
+    ref.close()
+    dflt = reference_run(ctrl, geo, 50)
+    assert not np.array_equal(body[2:2 + 38 * s1], dflt.download("pdf").ravel())
+    dflt.close()

@@ -409,3 +409,19 @@ def test_velocity_change_matches_host_norms(precision):
     s.upload("pdf", s.download("pdf"))
     assert s.velocity_change() == -1.0
     s.close()
+
+
+@pytest.mark.parametrize("model", [1, 3, 4])
+def test_collision_models_bitwise(model):
+    """The reference's other collision models (its compile-time `mrt` switch: 1 MRT
+    bounce-back optimised, 3 SRT, 4 MRT advection optimised), 40 steps of a drainage with
+    unequal viscosities, bit for bit against the oracle."""
+    geo, p = CASES["drainage_ywalls"]()
+    p = dict(p, mrt=model)
+    say(f"case=drainage_ywalls mrt={model}, 40 steps")
+    o, g = pair(geo, p)
+    o.setup()
+    g.setup()
+    o.run(1, 40)
+    g.run(1, 40)
+    compare_state(o, g, tag=f"mrt{model} step40")

@@ -46,6 +46,52 @@ def test_collide_node_matches_textbook_mrt(seed):
     np.testing.assert_allclose(o2, t2, rtol=0, atol=1e-14)
 
 
+@pytest.mark.parametrize("model", [1, 3, 4])
+@pytest.mark.parametrize("seed", range(3))
+def test_collision_models_match_textbook_mrt(model, seed):
+    """The reference's other collision models (its `mrt` macro: 1 bounce-back optimised,
+    3 SRT, 4 advection optimised) against the textbook matrix MRT with their rates."""
+    rng = np.random.default_rng(100 + seed)
+    p = base_params(lbm_gamma=rng.uniform(0.005, 0.05), force_z=rng.uniform(-1e-4, 1e-4),
+                    la_nui1=1 / rng.uniform(0.01, 0.2), la_nui2=1 / rng.uniform(0.01, 0.2),
+                    lbm_beta=rng.uniform(0.5, 0.99), mrt=model)
+    phi0 = rng.uniform(-1, 1)
+    rho = rng.uniform(0.95, 1.05)
+    u = rng.uniform(-0.05, 0.05, 3)
+    feq = TB.W * rho * (1 + 3 * TB.E @ u + 4.5 * (TB.E @ u) ** 2 - 1.5 * u @ u)
+    f = feq * (1 + rng.uniform(-0.02, 0.02, 19))
+    g1 = f * 0.5 * (1 + phi0)
+    g2 = f - g1
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    curv, cnorm = rng.uniform(-0.3, 0.3), rng.uniform(0, 0.5)
+    phi_o, o1, o2 = O.collide_node(p, g1, g2, n, curv, cnorm)
+    phi_t, t1, t2 = TB.collide(g1, g2, n, curv, cnorm, p["lbm_gamma"], p["force_z"], p["la_nui1"], p["la_nui2"],
+                               p["lbm_beta"], model=model)
+    assert abs(phi_o - phi_t) < 1e-14
+    np.testing.assert_allclose(o1, t1, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(o2, t2, rtol=0, atol=1e-14)
+    # the models differ from the default (mrt 2) on the same state
+    _, d1, _ = O.collide_node(dict(p, mrt=2), g1, g2, n, curv, cnorm)
+    assert np.abs(d1 - o1).max() > 1e-9
+
+
+def test_srt_model_is_bgk_towards_the_rest_state():
+    """Model 3 (SRT) relaxes every moment at omega, so at zero momentum and without force it
+    is f' = f - omega (f - rho r), r the collision's rest weights (4/21, 13/126, 1/63)."""
+    rng = np.random.default_rng(7)
+    nu = 0.08
+    p = base_params(lbm_gamma=0.0, force_z=0.0, la_nui1=1 / nu, la_nui2=1 / nu, mrt=3)
+    f = rng.uniform(0.01, 0.1, 19)
+    f = 0.5 * (f + f[TB.OPP])  # symmetric: zero momentum
+    rho = f.sum()
+    r = np.array([4 / 21] + [13 / 126] * 6 + [1 / 63] * 12)
+    omega = 1.0 / (3.0 * nu + 0.5)  # tau = 3 nu + 1/2
+    _, o1, o2 = O.collide_node(p, f, np.zeros(19), (0, 0, 0), 0.0, 0.0)
+    np.testing.assert_allclose(o1, f - omega * (f - rho * r), rtol=0, atol=2e-16 * 50)
+    assert np.all(o2 == 0)
+
+
 def test_mrt_rest_equilibrium_is_fixed_point():
     """The moment-space equilibrium at rest (eps = 0 because omega_eps = 0) is invariant."""
     p = base_params(lbm_gamma=0.0, force_z=0.0)
