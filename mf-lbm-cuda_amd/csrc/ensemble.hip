@@ -113,17 +113,8 @@ __global__ __launch_bounds__(EB) void k_ens_check(EnsDev E, float* __restrict__ 
         for (int q = 1; q < 19; ++q) r = r + ft[q];
         drho[t] = r;
     }
-    __shared__ double red[3][EB];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) red[m][threadIdx.x] = acc[m];
-    __syncthreads();
-    for (int s = EB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-#pragma unroll
-            for (int m = 0; m < 3; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) part[(int64_t)b * 3 + threadIdx.x] = red[threadIdx.x][0];
+    double s;
+    if (dev::block_tree<3, EB>(acc, s)) part[(int64_t)b * 3 + threadIdx.x] = s;
 }
 
 // One block per domain: its blocks' partial sums in a fixed order.
@@ -135,17 +126,8 @@ __global__ __launch_bounds__(EB) void k_ens_reduce(const double* __restrict__ pa
     for (int64_t b = threadIdx.x; b < nb; b += EB)
 #pragma unroll
         for (int m = 0; m < 3; ++m) acc[m] += part[(b0 + b) * 3 + m];
-    __shared__ double red[3][EB];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) red[m][threadIdx.x] = acc[m];
-    __syncthreads();
-    for (int s = EB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-#pragma unroll
-            for (int m = 0; m < 3; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) out[(int64_t)d * 3 + threadIdx.x] = red[threadIdx.x][0];
+    double s;
+    if (dev::block_tree<3, EB>(acc, s)) out[(int64_t)d * 3 + threadIdx.x] = s;
 }
 
 // Scatter the stored fields of domains first .. first+count-1 into dense arrays.

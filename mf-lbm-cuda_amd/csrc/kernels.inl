@@ -37,6 +37,41 @@ __device__ __forceinline__ int xcd_block() {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// FP64 block reductions as wavefront-shuffle trees.  Each wave combines its 64 lanes with
+// xor shuffles at distances 32, 16, ..., 1 (the two lanes of a pair compute the same
+// commutative sum, so every lane ends with the wave's value), lane 0 parks it in LDS, and
+// thread m combines the NT / 64 wave values of quantity m in wave order.  The pairing is
+// fixed, so the result never depends on timing.  MAXI is the one quantity (if any) combined
+// by fmax instead of +.  Thread m < NV receives quantity m in `out` and returns true; the
+// others return false.  One call per kernel (its LDS is not guarded for reuse).
+__device__ __forceinline__ double wave_tree(double v, bool is_max) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        const double o = __shfl_xor(v, m, 64);
+        v = is_max ? fmax(v, o) : v + o;
+    }
+    return v;
+}
+
+template <int NV, int NT, int MAXI = -1>
+__device__ __forceinline__ bool block_tree(double (&acc)[NV], double& out) {
+    static_assert(NT % 64 == 0 && NV <= NT, "block of whole waves");
+    __shared__ double wred[NV][NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+        const double v = wave_tree(acc[m], m == MAXI);
+        if (lane == 0) wred[m][wave] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= NV) return false;
+    const int m = threadIdx.x;
+    double s = wred[m][0];
+    for (int w = 1; w < NT / 64; ++w) s = (m == MAXI) ? fmax(s, wred[m][w]) : s + wred[m][w];
+    out = s;
+    return true;
+}
+
 template <typename T>
 struct W {
     static constexpr T w0 = T(1.0) / T(3.0);
@@ -883,21 +918,9 @@ __global__ __launch_bounds__(256) void k_monitor_partial(Sparse<T> S, StepConst<
         if (ph > 0.999) acc[8] += u2;
         else if (ph < -0.999) acc[9] += u2;
     }
-    __shared__ double red[MON_VALUES][256];
-#pragma unroll
-    for (int m = 0; m < MON_VALUES; ++m) red[m][threadIdx.x] = acc[m];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int m = 0; m < MON_VALUES; ++m)
-                red[m][threadIdx.x] = (m == 7) ? fmax(red[m][threadIdx.x], red[m][threadIdx.x + s])
-                                               : red[m][threadIdx.x] + red[m][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < MON_VALUES)
-        part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MON_VALUES + threadIdx.x] = red[threadIdx.x][0];
+    double s;
+    if (block_tree<MON_VALUES, 256, 7>(acc, s))
+        part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MON_VALUES + threadIdx.x] = s;
 }
 
 // Combine the partials of each slice in block order: out[m * nz + k].
@@ -933,18 +956,8 @@ __global__ __launch_bounds__(256) void k_velocity_change(Sparse<T> S, StepConst<
         prev[S.nf + t] = v;
         prev[2 * (int64_t)S.nf + t] = w;
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = d2;
-    red[1][threadIdx.x] = u2;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 2) part[2 * (int64_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    double acc[2] = {d2, u2}, s;
+    if (block_tree<2, 256>(acc, s)) part[2 * (int64_t)blockIdx.x + threadIdx.x] = s;
 }
 
 // out[0..1] = sums of the n pairs in part; one block, each thread a fixed stride.
@@ -955,18 +968,8 @@ __global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ pa
         a += part[2 * (int64_t)i];
         b += part[2 * (int64_t)i + 1];
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0];
+    double acc[2] = {a, b}, s;
+    if (block_tree<2, 256>(acc, s)) out[threadIdx.x] = s;
 }
 
 }  // namespace dev
