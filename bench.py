@@ -221,6 +221,42 @@ def measure_traffic(geo, args):
     return res
 
 
+def measure_precision(params, geo, n_all, prec, steps, warmup, profile_steps):
+    """Step rate and per-kernel times of the same workload in another precision (a fresh
+    handle: set-up, `warmup` untimed steps, `steps` timed ones), for the line's second-build
+    entry; the collide's fraction of the HBM peak on its algorithmic bytes as for the main
+    figure."""
+    s = 8 if prec == gpu.FP64 else 4
+    sim = gpu.Solver(params, geo, prec)
+    try:
+        sim.setup()
+        n_fluid = sim.counts()["fluid"]
+        sim.run(1, warmup)
+        t1 = 1 + warmup
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.run(t1, steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        kt = sim.profile(t1 + steps, profile_steps)
+        chain = sim.gradient_path()
+    finally:
+        sim.close()
+    ns = max(1, kt["n_odd"] + kt["n_even"])
+    odd = kt["collide_odd_ms"] / max(1, kt["n_odd"])
+    even = kt["collide_even_ms"] / max(1, kt["n_even"])
+    coll = 0.5 * (odd + even)
+    return {"dtype": "f64" if prec == gpu.FP64 else "f32", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(dt / steps * 1e3, 4), "mlups": round(n_all * steps / dt / 1e6, 3),
+            "fluid_glups": round(n_fluid * steps / dt / 1e9, 4),
+            "collide_roofline_frac": round(n_fluid * 77 * s / (coll * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_roofline_frac": round((n_fluid * 78 * s + n_all) / (dt / steps) / 1e9 / HBM_PEAK_GBS, 4),
+            "gradient_chain": chain,
+            "kernel_ms": {"collide_odd": round(odd, 4), "collide_even": round(even, 4),
+                          "boundary": round(kt["boundary_ms"] / ns, 4),
+                          "gradient": round(kt["gradient_ms"] / ns, 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +272,8 @@ def main():
     ap.add_argument("--probe-mib", type=int, default=1024, help="plain-copy calibration size (0 = skip)")
     ap.add_argument("--traffic", action=argparse.BooleanOptionalAction, default=True,
                     help="measure HBM bytes per kernel with two rocprofv3 --pmc passes")
+    ap.add_argument("--other-steps", type=int, default=1000,
+                    help="timed steps of the same workload in the other precision (0 = skip)")
     ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -351,6 +389,14 @@ def main():
     chain = sim.gradient_path()
     sim.close()
 
+    other = None
+    if args.other_steps > 0:
+        oprec = gpu.FP32 if prec == gpu.FP64 else gpu.FP64
+        log(f"the same workload in {'FP32' if oprec == gpu.FP32 else 'FP64'}: {args.other_steps} steps")
+        other = measure_precision(params, geo, n_all, oprec, args.other_steps, args.warmup, args.profile_steps)
+        log(f"{other['dtype']}: {other['mlups']:.1f} MLUPS, {other['ms_per_step']:.4f} ms per step, "
+            f"collide {other['collide_roofline_frac']:.3f} of the HBM peak")
+
     cpu = None
     if args.cpu_steps > 0:
         log(f"CPU baseline: {args.cpu_steps} oracle steps")
@@ -386,6 +432,7 @@ def main():
         "traffic_bytes": traffic,
         "copy_probe": probe,
         "cpu_baseline": cpu,
+        "other_precision": other,
     }
     print(json.dumps(out), flush=True)
 
