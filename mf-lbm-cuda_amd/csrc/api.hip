@@ -60,7 +60,7 @@ struct mflbm_handle {
     int32_t* zstart = nullptr;
     // fused gradient chain
     int32_t* tag = nullptr;     // S4 node tags (lattice.h GradArgs)
-    int grad_shape = 0, tiles_x = 0, tiles_y = 0, kchunk = 0;
+    int grad_shape = 0, tiles_x = 0, tiles_y = 0, kchunk = 0, box_order = 0;
     bool split_gradient = false;
     bool grad_dirty = false;    // dense phi / cn / c_norm / curv behind the step state
     // monitor scratch
@@ -143,6 +143,7 @@ GradArgs<T> grad_args(const mflbm_handle* h) {
     A.aux = (T*)h->aux;
     A.cnx = (T*)h->cnx; A.cny = (T*)h->cny; A.cnz = (T*)h->cnz; A.cnorm = (T*)h->cnorm;
     A.shape = h->grad_shape; A.tiles_x = h->tiles_x; A.tiles_y = h->tiles_y; A.kchunk = h->kchunk;
+    A.tiles_z = 1; A.box_order = h->box_order;
     return A;
 }
 
@@ -519,7 +520,7 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         // tiles launch thousands of small workgroups and win where the runs are short, and
         // in FP32, where their extra arithmetic (halo recomputation in three dimensions)
         // costs less.  Measured (FP64 / FP32 ms per launch): 256^3 march 0.651 / 0.495,
-        // tiles 0.816 / 0.427; 64^3 march 0.143 / 0.105, tiles 0.026 / 0.021.  FP32 marches
+        // tiles 0.774 / 0.421; 64^3 march 0.143 / 0.105, tiles 0.026 / 0.021.  FP32 marches
         // with 6 waves per workgroup (3 per SIMD, 5 % faster); FP64 with 4 (6 measured 60 %
         // slower).  MFLBM_GRAD_SHAPE (0-4 march shapes, 5 tiles) and MFLBM_GRAD_KCHUNK
         // (march z-run) override the choice (tuning).
@@ -539,6 +540,14 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         if (!shape_env && !kc_env && (p.precision == MFLBM_FP32 || kc < GRAD_MARCH_MIN_RUN))
             h->grad_shape = GRAD_SHAPE_TILES;
         h->kchunk = std::min(kc, d.nz);
+        // Two-pass chain: walk the boxes x fastest, then z, then y, so that the boxes above
+        // and below one another run close together and share their z halo in L2; the y halo
+        // (12 loaded rows for 8 owned) is the one re-read from HBM.  Against x, y, z at 256^3:
+        // FP32 pair 0.428 -> 0.422 ms, FP64 0.816 -> 0.774 ms.  MFLBM_BOX_ORDER (0: x, y, z;
+        // 1: x, z, y) overrides it (tuning).
+        const char* order_env = std::getenv("MFLBM_BOX_ORDER");
+        h->box_order = order_env ? std::atoi(order_env) : 1;
+        if (h->box_order < 0 || h->box_order > 1) return fail(h, "MFLBM_BOX_ORDER out of range");
     }
 
     // fluid-compact population cells

@@ -423,12 +423,17 @@ struct BoxLists {
 
 // Owned box (i0, j0, k0) + [0, TX) x [0, TY) x [0, TZ) of this workgroup: boxes laid out x
 // fastest, `tx_n` x `ty_n` per z layer, the first at (org, org, org).
-__device__ __forceinline__ void box_origin(int tx_n, int ty_n, int org, int TX, int TY, int TZ, int& i0, int& j0,
-                                           int& k0) {
+__device__ __forceinline__ void box_origin(int tx_n, int ty_n, int tz_n, int order, int org, int TX, int TY, int TZ,
+                                           int& i0, int& j0, int& k0) {
     const int b = xcd_block();
     i0 = org + (b % tx_n) * TX;
-    j0 = org + ((b / tx_n) % ty_n) * TY;
-    k0 = org + (b / (tx_n * ty_n)) * TZ;
+    if (order == 0) {
+        j0 = org + ((b / tx_n) % ty_n) * TY;
+        k0 = org + (b / (tx_n * ty_n)) * TZ;
+    } else {
+        k0 = org + ((b / tx_n) % tz_n) * TZ;
+        j0 = org + (b / (tx_n * tz_n)) * TY;
+    }
 }
 
 template <typename T>
@@ -457,7 +462,7 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_normals(GradArgs<T
     const Dims& d = A.d;
     const int tid = threadIdx.x, lane = tid & 63;
     int i0, j0, k0;
-    box_origin(A.tiles_x, A.tiles_y, -1, TX, TY, TZ, i0, j0, k0);
+    box_origin(A.tiles_x, A.tiles_y, A.tiles_z, A.box_order, -1, TX, TY, TZ, i0, j0, k0);
     // owned extent, clipped to [-1, n+2]
     const int ox = min(TX, d.nx + 3 - i0), oy = min(TY, d.ny + 3 - j0), oz = min(TZ, d.nz + 3 - k0);
     ws_table<T>(s_ws, tid);
@@ -569,7 +574,7 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_force(GradArgs<T> 
     const Dims& d = A.d;
     const int tid = threadIdx.x, lane = tid & 63;
     int i0, j0, k0;
-    box_origin(A.tiles_x, A.tiles_y, 1, TX, TY, TZ, i0, j0, k0);
+    box_origin(A.tiles_x, A.tiles_y, A.tiles_z, A.box_order, 1, TX, TY, TZ, i0, j0, k0);
     const int ox = min(TX, d.nx + 1 - i0), oy = min(TY, d.ny + 1 - j0), oz = min(TZ, d.nz + 1 - k0);
     ws_table<T>(s_ws, tid);
     if (tid == 0) s_cnt = 0;

@@ -142,6 +142,9 @@ struct GradArgs {
     T* cnz;
     T* cnorm;
     int shape, tiles_x, tiles_y, kchunk;
+    // two-pass chain: boxes along z, and the order in which the workgroups walk the boxes
+    // (0: x fastest, then y, then z; 1: x fastest, then z, then y)
+    int tiles_z, box_order;
 };
 
 // Workgroup shapes of the fused gradient kernel: owned nodes in x and y, threads.

@@ -92,11 +92,13 @@ void Launch<T>::gradient(const GradArgs<T>& A, const StepConst<T>& P, hipStream_
         // normals over [-1, n+2]^3, forces over [1, n]^3
         B.tiles_x = (A.d.nx + 4 + NX - 1) / NX;
         B.tiles_y = (A.d.ny + 4 + NY - 1) / NY;
-        const unsigned gn = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + 4 + NZ - 1) / NZ));
+        B.tiles_z = (A.d.nz + 4 + NZ - 1) / NZ;
+        const unsigned gn = (unsigned)(B.tiles_x * B.tiles_y * B.tiles_z);
         dev::k_grad_normals<T, NX, NY, NZ><<<gn, (NX + 4) * (NY + 4), 0, s>>>(B, P);
         B.tiles_x = (A.d.nx + FX - 1) / FX;
         B.tiles_y = (A.d.ny + FY - 1) / FY;
-        const unsigned gf = (unsigned)(B.tiles_x * B.tiles_y * ((A.d.nz + FZ - 1) / FZ));
+        B.tiles_z = (A.d.nz + FZ - 1) / FZ;
+        const unsigned gf = (unsigned)(B.tiles_x * B.tiles_y * B.tiles_z);
         dev::k_grad_force<T, FX, FY, FZ><<<gf, (FX + 4) * (FY + 4), 0, s>>>(B, P);
         return;
     }
