@@ -38,6 +38,9 @@ struct mflbm_handle {
     // transfer/background copy of the populations; the step works on the compact cells.
     void *pdf = nullptr, *phi = nullptr, *cnx = nullptr, *cny = nullptr, *cnz = nullptr, *cnorm = nullptr,
          *curv = nullptr, *snx = nullptr, *sny = nullptr, *snz = nullptr;
+    // the two-pass gradient chain's intermediate: (cn_x, cn_y, cn_z, c_norm) of each S2 node
+    // side by side (4 * S2 elements), written by its first kernel and read by its second
+    void* cnp = nullptr;
     uint8_t* walls = nullptr;
     int8_t* wtype = nullptr;
     uint32_t *l_fluid = nullptr, *l_fluid_ext = nullptr, *l_sb = nullptr;
@@ -142,6 +145,7 @@ GradArgs<T> grad_args(const mflbm_handle* h) {
     A.snx = (const T*)h->snx; A.sny = (const T*)h->sny; A.snz = (const T*)h->snz;
     A.aux = (T*)h->aux;
     A.cnx = (T*)h->cnx; A.cny = (T*)h->cny; A.cnz = (T*)h->cnz; A.cnorm = (T*)h->cnorm;
+    A.cnp = (T*)h->cnp;
     A.shape = h->grad_shape; A.tiles_x = h->tiles_x; A.tiles_y = h->tiles_y; A.kchunk = h->kchunk;
     A.tiles_z = 1; A.box_order = h->box_order;
     return A;
@@ -233,14 +237,14 @@ int materialize_gradient(mflbm_handle* h) {
 void free_all(mflbm_handle* h) {
     for (int g = 0; g < 4; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
-    void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->curv, h->snx, h->sny, h->snz,
+    void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->cnp, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
                     h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
                     h->ext_node, h->ext_off, h->zstart,
                     h->tag, h->macro_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->curv = h->snx = h->sny = h->snz = nullptr;
+    h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->cnp = h->curv = h->snx = h->sny = h->snz = nullptr;
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
     h->vprev = h->vpart = nullptr;
@@ -548,6 +552,10 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         const char* order_env = std::getenv("MFLBM_BOX_ORDER");
         h->box_order = order_env ? std::atoi(order_env) : 1;
         if (h->box_order < 0 || h->box_order > 1) return fail(h, "MFLBM_BOX_ORDER out of range");
+        if (h->grad_shape == GRAD_SHAPE_TILES) {
+            HIPCHK(h, hipMalloc(&h->cnp, 4 * d.S2 * esize(h)));
+            HIPCHK(h, hipMemsetAsync(h->cnp, 0, 4 * d.S2 * esize(h), h->stream));
+        }
     }
 
     // fluid-compact population cells

@@ -383,8 +383,9 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
 // dependent stages (about 5 us a plane at two workgroups per CU whatever the tile area),
 // so its time is set by latency where its z-runs are short.  Here the chain is cut where
 // the normals are complete: k_grad_normals computes K3 and K4+K5 on a box of nodes and
-// writes the normals and c_norm of its fluid nodes to the dense cn arrays; k_grad_force
-// reads them back, extrapolates them to the solid-boundary nodes (K6) and computes the
+// writes the normals and c_norm of its fluid nodes to a dense S2 array of 4-vectors
+// (Vec4: the four values of a node side by side, one vector store); k_grad_force reads
+// them back, extrapolates them to the solid-boundary nodes (K6) and computes the
 // curvature and force records (K7).  Each box is loaded once with a halo of two nodes,
 // every workgroup runs three short phases, and many small workgroups per CU hide the
 // latency.  Node sets and arithmetic are those of the march: K3 over [-2, n+3], K4 over
@@ -393,6 +394,13 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
 // A workgroup has one thread per column (x, y) of its box (LX * LY threads); the thread
 // walks its column up z, so that its loads, LDS stores and list flags take one address
 // step per plane.  Box entry e = x + LX * (y + LY * z).
+
+// (cn_x, cn_y, cn_z, c_norm) of one node side by side, so that one vector access moves
+// all four (16 B in FP32, 32 B in FP64) and a row of a box is one contiguous run.
+template <typename T>
+struct alignas(4 * sizeof(T)) Vec4 {
+    T x, y, z, w;
+};
 
 // Stage lists of a box, built from the tag words each thread loaded: a wave counts its
 // entries with ballots and reserves its runs with one LDS atomic on the packed counts.
@@ -553,7 +561,7 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_normals(GradArgs<T
             cx, cy, cz, wet, P.cos_theta, [&](T& nwx, T& nwy, T& nwz) { nwx = wn[m][0]; nwy = wn[m][1]; nwz = wn[m][2]; },
             vx, vy, vz);
         const int c2 = i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC);
-        A.cnx[c2] = vx; A.cny[c2] = vy; A.cnz[c2] = vz; A.cnorm[c2] = cnorm;
+        reinterpret_cast<Vec4<T>*>(A.cnp)[c2] = Vec4<T>{vx, vy, vz, cnorm};
     }
 }
 
@@ -597,7 +605,8 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_force(GradArgs<T> 
         const int zz = min(z, zmax);
         const int64_t c2 = n0 + (int64_t)zz * P2;
         tv[z] = A.tag[t0 + (int64_t)zz * P4];
-        cv[z][0] = A.cnx[c2]; cv[z][1] = A.cny[c2]; cv[z][2] = A.cnz[c2];
+        const Vec4<T> v = reinterpret_cast<const Vec4<T>*>(A.cnp)[c2];
+        cv[z][0] = v.x; cv[z][1] = v.y; cv[z][2] = v.z;
         if (!col || z > zmax) tv[z] = TAG_SOLID;
     }
     __syncthreads();
@@ -626,7 +635,7 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_force(GradArgs<T> 
         cnv[m] = T(0);
         if (i < n7) {
             const int e = s_l7[i];
-            cnv[m] = A.cnorm[i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC)];
+            cnv[m] = A.cnp[4 * (int64_t)i2(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC) + 3];
         }
     }
 

@@ -137,10 +137,11 @@ struct GradArgs {
     const T* __restrict__ sny;
     const T* __restrict__ snz;
     T* aux;                           // 4 * nf force records
-    T* cnx;                           // S2 dense normals and c_norm (two-pass chain)
+    T* cnx;                           // S2 dense normals and c_norm
     T* cny;
     T* cnz;
     T* cnorm;
+    T* cnp;                           // 4 * S2: (cn_x, cn_y, cn_z, c_norm) side by side (two-pass chain)
     int shape, tiles_x, tiles_y, kchunk;
     // two-pass chain: boxes along z, and the order in which the workgroups walk the boxes
     // (0: x fastest, then y, then z; 1: x fastest, then z, then y)
