@@ -41,6 +41,10 @@ struct mflbm_handle {
     // the two-pass gradient chain's intermediate: (cn_x, cn_y, cn_z, c_norm) of each S2 node
     // side by side (4 * S2 elements), written by its first kernel and read by its second
     void* cnp = nullptr;
+    // the solid normals as 4-vectors (snx, sny, snz, 0) per S4 node: what the two-pass
+    // gradient chain reads at wetting nodes (one vector load); rebuilt whenever snx/sny/snz
+    // change
+    void* snp = nullptr;
     uint8_t* walls = nullptr;
     int8_t* wtype = nullptr;
     uint32_t *l_fluid = nullptr, *l_fluid_ext = nullptr, *l_sb = nullptr;
@@ -143,6 +147,7 @@ GradArgs<T> grad_args(const mflbm_handle* h) {
     A.phi = (T*)h->phi;
     A.tag = h->tag;
     A.snx = (const T*)h->snx; A.sny = (const T*)h->sny; A.snz = (const T*)h->snz;
+    A.snp = (const T*)h->snp;
     A.aux = (T*)h->aux;
     A.cnx = (T*)h->cnx; A.cny = (T*)h->cny; A.cnz = (T*)h->cnz; A.cnorm = (T*)h->cnorm;
     A.cnp = (T*)h->cnp;
@@ -234,17 +239,27 @@ int materialize_gradient(mflbm_handle* h) {
     return 0;
 }
 
+// The gradient chain's copy of the solid normals, (snx, sny, snz, 0) per S4 node.
+void pack_solid_normals(mflbm_handle* h) {
+    if (h->prec == MFLBM_FP64)
+        Launch<double>::pack4((const double*)h->snx, (const double*)h->sny, (const double*)h->snz, (double*)h->snp,
+                              h->d.S4, h->stream);
+    else
+        Launch<float>::pack4((const float*)h->snx, (const float*)h->sny, (const float*)h->snz, (float*)h->snp, h->d.S4,
+                             h->stream);
+}
+
 void free_all(mflbm_handle* h) {
     for (int g = 0; g < 4; ++g)
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
-    void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->cnp, h->curv, h->snx, h->sny, h->snz,
+    void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->cnp, h->snp, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
                     h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
                     h->ext_node, h->ext_off, h->zstart,
                     h->tag, h->macro_buf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->cnp = h->curv = h->snx = h->sny = h->snz = nullptr;
+    h->pdf = h->phi = h->cnx = h->cny = h->cnz = h->cnorm = h->cnp = h->snp = h->curv = h->snx = h->sny = h->snz = nullptr;
     h->walls = nullptr; h->wtype = nullptr; h->l_fluid = h->l_fluid_ext = h->l_sb = nullptr;
     h->mon_part = h->mon_out = nullptr;
     h->vprev = h->vpart = nullptr;
@@ -461,6 +476,8 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         HIPCHK(h, hipStreamSynchronize(h->stream));
         (void)hipFree(sx); (void)hipFree(sy); (void)hipFree(sz);
     }
+    HIPCHK(h, hipMalloc(&h->snp, 4 * d.S4 * es));
+    pack_solid_normals(h);
 
     // node lists from the classified geometry
     std::vector<uint8_t> walls(d.S2);
@@ -808,6 +825,7 @@ int mflbm_upload(mflbm_handle* h, int32_t field, const void* src, size_t bytes) 
     }
     if (field == MFLBM_CN_X || field == MFLBM_CN_Y || field == MFLBM_CN_Z || field == MFLBM_C_NORM || field == MFLBM_CURV)
         aux_refresh(h);
+    if (field == MFLBM_S_NX || field == MFLBM_S_NY || field == MFLBM_S_NZ) pack_solid_normals(h);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;

@@ -42,6 +42,14 @@ __device__ __forceinline__ double keep_if(bool f, double v) {
 }
 __device__ __forceinline__ float keep_if(bool f, float v) { return __int_as_float(__float_as_int(v) & (f ? ~0 : 0)); }
 
+// Four values of one node side by side, so that one vector access moves all four (16 B in
+// FP32, 32 B in FP64) and a row of nodes is one contiguous run: the two-pass chain's
+// (cn_x, cn_y, cn_z, c_norm) and its solid normals (snx, sny, snz, 0) of wetting nodes.
+template <typename T>
+struct alignas(4 * sizeof(T)) Vec4 {
+    T x, y, z, w;
+};
+
 // Workgroup barrier for data exchanged through LDS only.  __syncthreads() also waits for
 // every outstanding global load of the wave (its release fence covers all memory), which
 // would expose the latency of the next plane's loads at every stage boundary.
@@ -245,6 +253,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
                 if (i < n) {
                     const int v = list[i], e = pos(v);
                     if (v & (1 << 14)) {
+                        // three scalar loads: the 4-vector copy measured no faster here (FP64)
                         const int c = i4(d, ib + e % WX, jb + e / WX, q);
                         wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
                     }
@@ -395,13 +404,6 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_gradient(GradArgs<T> A, StepCo
 // walks its column up z, so that its loads, LDS stores and list flags take one address
 // step per plane.  Box entry e = x + LX * (y + LY * z).
 
-// (cn_x, cn_y, cn_z, c_norm) of one node side by side, so that one vector access moves
-// all four (16 B in FP32, 32 B in FP64) and a row of a box is one contiguous run.
-template <typename T>
-struct alignas(4 * sizeof(T)) Vec4 {
-    T x, y, z, w;
-};
-
 // Stage lists of a box, built from the tag words each thread loaded: a wave counts its
 // entries with ballots and reserves its runs with one LDS atomic on the packed counts.
 struct BoxLists {
@@ -517,7 +519,8 @@ __global__ __launch_bounds__((TX + 4) * (TY + 4)) void k_grad_normals(GradArgs<T
             const int v = s_l4[i], e = v & 0xFFF;
             if ((v >> 12) & 1) {
                 const int c = i4(d, i0 - 2 + e % LX, j0 - 2 + (e / LX) % LY, k0 - 2 + e / NC);
-                wn[m][0] = A.snx[c]; wn[m][1] = A.sny[c]; wn[m][2] = A.snz[c];
+                const Vec4<T> sn = reinterpret_cast<const Vec4<T>*>(A.snp)[c];
+                wn[m][0] = sn.x; wn[m][1] = sn.y; wn[m][2] = sn.z;
             }
         }
     }

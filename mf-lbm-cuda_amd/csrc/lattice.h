@@ -136,6 +136,7 @@ struct GradArgs {
     const T* __restrict__ snx;        // S4 solid normals, read at fluid-boundary nodes
     const T* __restrict__ sny;
     const T* __restrict__ snz;
+    const T* __restrict__ snp;        // 4 * S4: (snx, sny, snz, 0) side by side (two-pass chain)
     T* aux;                           // 4 * nf force records
     T* cnx;                           // S2 dense normals and c_norm
     T* cny;

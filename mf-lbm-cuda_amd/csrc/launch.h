@@ -36,6 +36,8 @@ struct Launch {
     static void monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi, const Dims& d, const int32_t* zstart,
                         int nsplit, double* part, double* out, hipStream_t s);
     static void to_real(const double* src, T* dst, int64_t n, hipStream_t s);
+    // dst[4 i .. 4 i + 3] = (x[i], y[i], z[i], 0)
+    static void pack4(const T* x, const T* y, const T* z, T* dst, int64_t n, hipStream_t s);
 };
 
 extern template struct Launch<double>;

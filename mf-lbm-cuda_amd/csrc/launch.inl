@@ -10,6 +10,17 @@ namespace {
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_pack4(const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+                                               T* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    dst[4 * i] = x[i];
+    dst[4 * i + 1] = y[i];
+    dst[4 * i + 2] = z[i];
+    dst[4 * i + 3] = T(0);
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_to_real(const double* __restrict__ src, T* __restrict__ dst, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t < n) dst[t] = (T)src[t];
@@ -171,6 +182,11 @@ void Launch<T>::monitor(const Sparse<T>& S, const StepConst<T>& P, const T* phi,
     dim3 grid(nsplit, d.nz);
     dev::k_monitor_partial<T><<<grid, 256, 0, s>>>(S, P, phi, d, zstart, part);
     dev::k_monitor_combine<T><<<blocks(d.nz, 64), 64, 0, s>>>(part, nsplit, d.nz, out);
+}
+
+template <typename T>
+void Launch<T>::pack4(const T* x, const T* y, const T* z, T* dst, int64_t n, hipStream_t s) {
+    k_pack4<T><<<blocks(n), 256, 0, s>>>(x, y, z, dst, n);
 }
 
 template <typename T>
