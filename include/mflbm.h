@@ -165,7 +165,9 @@ int mflbm_sync(mflbm_handle* h);
 
 /* Node counts: 0 fluid interior, 1 fluid in [-1,n+2], 2 solid-boundary in [-2,n+3].
  * which = 3 names the gradient chain the step runs: 0-4 the z-march in one of its workgroup
- * shapes, 5 the two-pass 3-D tiles, 6 the four split kernels. */
+ * shapes, 5 the two-pass 3-D tiles, 6 the four split kernels.  which = 4: the waves of 64
+ * fluid nodes whose links span too many cells for the 16-bit neighbour rows and read the
+ * 32-bit ones. */
 int64_t mflbm_count(const mflbm_handle* h, int32_t which);
 
 /* Size in elements of a field. */

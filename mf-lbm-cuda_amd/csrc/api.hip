@@ -54,6 +54,9 @@ struct mflbm_handle {
     int64_t ns = 0;
     void* aux = nullptr;        // 4 * n_fluid force records
     int32_t* nb = nullptr;      // 18 * n_fluid
+    uint32_t* nb16 = nullptr;   // 9 * n_fluid: short neighbour rows
+    int32_t* nbb = nullptr;     // NB_WAVE per wave of 64 fluid nodes
+    int n_escaped = 0;          // waves that keep their 32-bit neighbour rows
     int32_t* pairs[2] = {nullptr, nullptr};
     int npairs[2] = {0, 0};
     int32_t* zmap = nullptr;
@@ -135,6 +138,8 @@ Sparse<T> sparse(const mflbm_handle* h) {
     S.ns = h->ns;
     S.nf = h->n_fluid;
     S.nb = h->nb;
+    S.nb16 = h->nb16;
+    S.nbb = h->nbb;
     S.list = h->l_fluid;
     S.aux = (T*)h->aux;
     return S;
@@ -254,7 +259,7 @@ void free_all(mflbm_handle* h) {
         if (h->graph[g]) { (void)hipGraphExecDestroy(h->graph[g]); h->graph[g] = nullptr; }
     void* ptrs[] = {h->pdf, h->phi, h->cnx, h->cny, h->cnz, h->cnorm, h->cnp, h->snp, h->curv, h->snx, h->sny, h->snz,
                     h->walls, h->wtype, h->l_fluid, h->l_fluid_ext, h->l_sb, h->mon_part, h->mon_out, h->vprev, h->vpart,
-                    h->cells, h->aux, h->nb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
+                    h->cells, h->aux, h->nb, h->nb16, h->nbb, h->pairs[0], h->pairs[1], h->zmap, h->pmap, h->w_in, h->conv,
                     h->ext_node, h->ext_off, h->zstart,
                     h->tag, h->macro_buf};
     for (void* p : ptrs)
@@ -265,6 +270,8 @@ void free_all(mflbm_handle* h) {
     h->vprev = h->vpart = nullptr;
     h->macro_buf = nullptr;
     h->cells = h->aux = nullptr;
+    h->nb16 = nullptr;
+    h->nbb = nullptr;
     h->nb = h->pairs[0] = h->pairs[1] = h->zmap = h->pmap = h->ext_node = h->ext_off = h->zstart = nullptr;
     h->w_in = h->conv = nullptr;
     h->tag = nullptr;
@@ -594,6 +601,16 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         return hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     };
     HIPCHK(h, upload_i32(&h->nb, lay.nb));
+    // MFLBM_NB_SHORT=0 keeps every wave on the 32-bit rows (tuning)
+    if (const char* e = std::getenv("MFLBM_NB_SHORT"); e && std::atoi(e) == 0) {
+        for (size_t w = 0; w < lay.nbb.size() / NB_WAVE; ++w) lay.nbb[w * NB_WAVE + 36] = 1;
+        lay.n_escaped = (int)(lay.nbb.size() / NB_WAVE);
+    }
+    h->n_escaped = lay.n_escaped;
+    HIPCHK(h, upload_i32(&h->nbb, lay.nbb));
+    HIPCHK(h, hipMalloc(&h->nb16, std::max<size_t>(1, lay.nb16.size()) * sizeof(uint32_t)));
+    if (!lay.nb16.empty())
+        HIPCHK(h, hipMemcpy(h->nb16, lay.nb16.data(), lay.nb16.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(h, upload_i32(&h->ext_node, lay.ext_node));
     HIPCHK(h, upload_i32(&h->ext_off, lay.ext_off));
     HIPCHK(h, upload_i32(&h->pairs[0], lay.pairs[0]));
@@ -913,6 +930,7 @@ int mflbm_sync(mflbm_handle* h) {
 int64_t mflbm_count(const mflbm_handle* h, int32_t which) {
     if (!h) return -1;
     if (which == 3) return h->split_gradient ? GRAD_SHAPE_TILES + 1 : h->grad_shape;
+    if (which == 4) return h->n_escaped;
     return which == 0 ? h->n_fluid : which == 1 ? h->n_fluid_ext : which == 2 ? h->n_sb : -1;
 }
 

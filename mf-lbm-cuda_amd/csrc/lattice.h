@@ -104,6 +104,8 @@ struct ShiftedStorage { static constexpr bool value = false; };
 template <>
 struct ShiftedStorage<float> { static constexpr bool value = true; };
 
+constexpr int NB_WAVE = 40;  // words per wave of the short neighbour rows (36 bases, escape flag, padding)
+
 // Fluid-compact population storage (see kernels.inl for the cell layout).
 template <typename T>
 struct Sparse {
@@ -111,6 +113,14 @@ struct Sparse {
     int64_t ns;                     // cells (a multiple of CELL_BLOCK)
     int nf;                         // interior fluid nodes
     const int32_t* __restrict__ nb; // 18 * nf
+    // The same links in half the bytes.  A wave of 64 nodes (t / 64) holds, per direction q,
+    // the base cell of the fluid cells (class 0, cell < nf) and of the extra cells (class 1)
+    // it links to: nbb[w * NB_WAVE + 2 (q - 1) + class]; nbb[w * NB_WAVE + 36] != 0 marks a
+    // wave whose links of one class span 2^15 cells or more, which reads nb instead.  Each
+    // node holds, per q, (cell - base) | class << 15 as 16 bits, directions 2p + 1 and 2p + 2
+    // in the low and high half of nb16[p * nf + t].
+    const uint32_t* __restrict__ nb16;  // 9 * nf
+    const int32_t* __restrict__ nbb;    // NB_WAVE * ceil(nf / 64)
     const uint32_t* __restrict__ list;  // nf packed coordinates
     T* aux;                         // 4 * nf force records (cn_x, cn_y, cn_z, 0.5 gamma curv c_norm)
     // Position of slot s (0 .. 37) of cell c in f.  Cells come in blocks of CELL_BLOCK, each

@@ -11,6 +11,11 @@ struct LayoutHost {
     int nf = 0;                     // interior fluid nodes
     int64_t ns = 0;                 // cells per slot array (nf + max extra cells of a slot)
     std::vector<int32_t> nb;        // 18 * nf
+    // short neighbour rows (Sparse<T>::nb16 / nbb): 9 * nf pairs of 16-bit links, NB_WAVE
+    // words per wave of 64 nodes; n_escaped waves keep their 32-bit rows
+    std::vector<uint32_t> nb16;
+    std::vector<int32_t> nbb;
+    int n_escaped = 0;
     std::vector<int32_t> ext_node;  // s1 node of every extra cell, grouped by slot
     std::vector<int32_t> ext_off;   // 20 offsets into ext_node
     std::vector<int32_t> pairs[2];  // periodic copies (slot, dst, src) for [0] even, [1] odd steps

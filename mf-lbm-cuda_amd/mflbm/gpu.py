@@ -275,7 +275,7 @@ class Solver:
     def counts(self):
         L = load()
         return {"fluid": L.mflbm_count(self.h, 0), "fluid_ext": L.mflbm_count(self.h, 1),
-                "solid_boundary": L.mflbm_count(self.h, 2)}
+                "solid_boundary": L.mflbm_count(self.h, 2), "long_link_waves": L.mflbm_count(self.h, 4)}
 
     def gradient_path(self):
         """'march' (one-pass z-march), 'tiles' (two-pass 3-D tiles) or 'split' (four kernels)"""

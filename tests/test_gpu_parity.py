@@ -113,6 +113,31 @@ def test_setup_and_steps_bitwise(name):
     compare_state(o, g, tag=f"{name} step40")
 
 
+def test_long_link_waves_bitwise():
+    """Two solid plates across a 256 x 256 plane between fully fluid layers, each with two
+    holes near opposite corners: the two holes of a plate are consecutive fluid nodes, so
+    the wave holding them links to fluid cells a whole plane apart, more than the 16-bit
+    neighbour rows span, and reads the 32-bit rows while every other wave decodes the
+    16-bit ones.  Both kinds match the oracle bit for bit."""
+    geo, p = small_drainage(n=(256, 256, 12), seed=3, jper=1, buffer=6)
+    geo[3] = 1
+    geo[3, 1, 1] = geo[3, 254, 254] = 0
+    geo[8] = 1
+    geo[8, 2, 3] = geo[8, 253, 250] = 0
+    say("case=long_link_waves 256x256x12 fp64")
+    o, g = pair(geo, p)
+    assert g.counts()["long_link_waves"] >= 1
+    o.setup()
+    g.setup()
+    for t in range(1, 5):
+        o.step(t)
+        g.step(t)
+    compare_state(o, g, tag="long links step4")
+    o.run(5, 8)
+    g.run(5, 8)
+    compare_state(o, g, tag="long links step12")
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_fused_gradient_matches_split(name):
     """The fused gradient-chain kernel and the four split kernels give identical steps.
