@@ -24,6 +24,7 @@
 #include "../../include/mflbm.h"
 #include "device_lock.h"
 #include "kernels.inl"
+#include "layout.h"
 
 using namespace mflbm;
 
@@ -36,6 +37,8 @@ struct EnsDev {
     int64_t ns;                // cells per slot array
     int64_t nf;                // padded fluid cells of all domains (multiple of EB)
     const int32_t* nb;         // 18 * nf: cell of (x + e_q) in slot array opp(q)
+    const uint32_t* nb16;      // the same links as 16-bit rows against per-wave bases (Sparse<T>)
+    const int32_t* nbb;
     const int32_t* blk_dom;    // domain of each block
     const int32_t* dom_start;  // first fluid cell of each domain
     const int32_t* dom_count;  // fluid nodes of each domain
@@ -57,8 +60,7 @@ __global__ __launch_bounds__(EB) void k_ens_collide(EnsDev E) {
     int64_t nb[19];
     if (ODD) {
         nb[0] = t;
-#pragma unroll
-        for (int q = 1; q < 19; ++q) nb[q] = __builtin_nontemporal_load(E.nb + (int64_t)(q - 1) * NF + t);
+        dev::odd_links(E.nbb, E.nb16, E.nb, NF, t, nb);
     }
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
@@ -180,6 +182,9 @@ struct mflbm_ensemble {
     int64_t nf = 0, ns = 0, nf_real = 0;
     std::vector<int32_t> start, count;
     float* f = nullptr;
+    uint32_t* nb16 = nullptr;
+    int32_t* nbb = nullptr;
+    int n_escaped = 0;  // waves on the 32-bit rows
     int32_t *nb = nullptr, *blk_dom = nullptr, *dom_start = nullptr, *dom_count = nullptr, *active = nullptr;
     int32_t* loc = nullptr;
     float *ux = nullptr, *uy = nullptr, *uz = nullptr, *drho = nullptr;
@@ -215,7 +220,7 @@ int fail(mflbm_ensemble* e, const char* fmt, ...) {
 
 EnsDev dev_view(const mflbm_ensemble* e) {
     EnsDev E;
-    E.f = e->f; E.ns = e->ns; E.nf = e->nf; E.nb = e->nb; E.blk_dom = e->blk_dom;
+    E.f = e->f; E.ns = e->ns; E.nf = e->nf; E.nb = e->nb; E.nb16 = e->nb16; E.nbb = e->nbb; E.blk_dom = e->blk_dom;
     E.dom_start = e->dom_start; E.dom_count = e->dom_count; E.active = e->active;
     E.nui = (float)(1.0 / e->nu);
     E.force = (float)e->force;
@@ -226,11 +231,12 @@ EnsDev dev_view(const mflbm_ensemble* e) {
 void free_all(mflbm_ensemble* e) {
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
     e->graph = nullptr;
-    void* p[] = {e->f, e->nb, e->blk_dom, e->dom_start, e->dom_count, e->active, e->loc, e->ux, e->uy, e->uz,
+    void* p[] = {e->f, e->nb, e->nb16, e->nbb, e->blk_dom, e->dom_start, e->dom_count, e->active, e->loc, e->ux, e->uy, e->uz,
                  e->drho, e->part, e->red};
     for (void* x : p)
         if (x) (void)hipFree(x);
-    e->f = nullptr; e->nb = e->blk_dom = e->dom_start = e->dom_count = e->active = e->loc = nullptr;
+    e->f = nullptr; e->nb16 = nullptr; e->nbb = nullptr;
+    e->nb = e->blk_dom = e->dom_start = e->dom_count = e->active = e->loc = nullptr;
     e->ux = e->uy = e->uz = e->drho = nullptr; e->part = e->red = nullptr;
     if (e->stream) (void)hipStreamDestroy(e->stream);
     e->stream = nullptr;
@@ -370,6 +376,20 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
         const int64_t nblk = (cnt[d].nf + EB - 1) / EB;
         for (int64_t b = 0; b < nblk; ++b) blk[(size_t)(base / EB + b)] = d;
     });
+    // short neighbour rows, domain by domain (a wave never spans two domains: their ranges
+    // are padded to whole blocks); the padding nodes are not encoded
+    std::vector<uint32_t> nb16((size_t)9 * nf, 0u);
+    std::vector<int32_t> nbb((size_t)(nf / 64) * NB_WAVE, 0);
+    std::vector<int> esc(nd, 0);
+    parallel([&](int d) {
+        const int64_t b = e->start[d], end = b + e->count[d];
+        const int64_t w0 = b / 64, w1 = (b + (e->count[d] + EB - 1) / EB * EB) / 64;
+        esc[d] = encode_short_rows(nb.data(), nf, w0, w1, [&](int64_t t) { return t < end; }, nb16.data(), nbb.data());
+    });
+    for (int d = 0; d < nd; ++d) {
+        if (esc[d] < 0) return fail(e, "ensemble: short neighbour row of domain %d does not decode", d);
+        e->n_escaped += esc[d];
+    }
 
     ECHK(e, hipStreamCreate(&e->stream));
     auto up = [&](int32_t** dst, const std::vector<int32_t>& v) -> hipError_t {
@@ -378,6 +398,9 @@ int mflbm_ensemble_create(const mflbm_params* params, const int8_t* geometries, 
         return hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     };
     ECHK(e, up(&e->nb, nb));
+    ECHK(e, up(&e->nbb, nbb));
+    ECHK(e, hipMalloc(&e->nb16, nb16.size() * sizeof(uint32_t)));
+    ECHK(e, hipMemcpy(e->nb16, nb16.data(), nb16.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     ECHK(e, up(&e->loc, loc));
     ECHK(e, up(&e->blk_dom, blk));
     ECHK(e, up(&e->dom_start, e->start));

@@ -300,6 +300,41 @@ __device__ __forceinline__ T collide_node(const StepConst<T>& P, const T (&g1)[1
 // odd, 395-726 even).  Odd: pull q from (x - e_q).slot q, push q to (x + e_q).slot opp(q);
 // even: read q from x.slot opp(q), write q to x.slot q.  Each cell is read and written by
 // exactly one node per sweep, so the update is in place.
+// Links of node t (cells of x + e_q, q = 1..18, into nb[1..18]) from the short neighbour rows
+// (Sparse<T>::nb16 / nbb): the wave's bases, one address for the whole wave (scalar loads),
+// and the node's nine pairs of 16-bit links; a flagged wave reads the 32-bit rows nb32
+// instead.  38.5 bytes per node against 72.  t / 64 must be the wave's index, i.e. nodes
+// are mapped to lanes in order.  The loads are issued, and held in front of what follows,
+// before the links are decoded.
+template <typename I>
+__device__ __forceinline__ void odd_links(const int32_t* __restrict__ nbb, const uint32_t* __restrict__ nb16,
+                                          const int32_t* __restrict__ nb32, int64_t nf, int64_t t, I (&nb)[19]) {
+    const int w = __builtin_amdgcn_readfirstlane((int)(t >> 6));
+    const int4* wb = reinterpret_cast<const int4*>(nbb + (int64_t)w * NB_WAVE);
+    int4 bq[NB_WAVE / 4];
+#pragma unroll
+    for (int m = 0; m < NB_WAVE / 4; ++m) bq[m] = wb[m];
+    uint32_t pr[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) pr[p] = __builtin_nontemporal_load(nb16 + (int64_t)p * nf + t);
+    __builtin_amdgcn_sched_barrier(0);
+    if (bq[9].x) {  // a wave whose links span too many cells for 15 bits
+#pragma unroll
+        for (int q = 1; q < 19; ++q) nb[q] = (I)__builtin_nontemporal_load(nb32 + (int64_t)(q - 1) * nf + t);
+    } else {
+        auto word = [&](int k) {
+            const int4& v = bq[k >> 2];
+            return (k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w;
+        };
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            const uint32_t u = ((q - 1) & 1) ? pr[(q - 1) >> 1] >> 16 : pr[(q - 1) >> 1] & 0xFFFFu;
+            const int k = 2 * (q - 1);
+            nb[q] = (I)(((u >> 15) ? word(k + 1) : word(k)) + (int)(u & 0x7FFFu));
+        }
+    }
+}
+
 #ifndef MFLBM_COLLIDE_THREADS
 #define MFLBM_COLLIDE_THREADS 256  // threads per collide workgroup (tuning: 128, 256, 512)
 #endif
@@ -325,32 +360,7 @@ __global__ __launch_bounds__(MFLBM_COLLIDE_THREADS) void k_collide(Sparse<T> S, 
         for (int m = 0; m < 4; ++m) rec[m] = __builtin_nontemporal_load(a + m);
         packed = S.list[t];
         nb[0] = t;
-        // the short neighbour rows: the wave's bases (one address for the whole wave) and the
-        // node's nine pairs of 16-bit links, 38.5 bytes per node against 72 for the 32-bit rows
-        const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-        const int4* wb = reinterpret_cast<const int4*>(S.nbb + (int64_t)w * NB_WAVE);
-        int4 bq[NB_WAVE / 4];
-#pragma unroll
-        for (int m = 0; m < NB_WAVE / 4; ++m) bq[m] = wb[m];
-        uint32_t pr[9];
-#pragma unroll
-        for (int p = 0; p < 9; ++p) pr[p] = __builtin_nontemporal_load(S.nb16 + (int64_t)p * S.nf + t);
-        __builtin_amdgcn_sched_barrier(0);
-        if (bq[9].x) {  // a wave whose links span too many cells for 15 bits
-#pragma unroll
-            for (int q = 1; q < 19; ++q) nb[q] = __builtin_nontemporal_load(S.nb + (int64_t)(q - 1) * S.nf + t);
-        } else {
-            auto word = [&](int k) {
-                const int4& v = bq[k >> 2];
-                return (k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w;
-            };
-#pragma unroll
-            for (int q = 1; q < 19; ++q) {
-                const uint32_t u = ((q - 1) & 1) ? pr[(q - 1) >> 1] >> 16 : pr[(q - 1) >> 1] & 0xFFFFu;
-                const int k = 2 * (q - 1);
-                nb[q] = ((u >> 15) ? word(k + 1) : word(k)) + (int)(u & 0x7FFFu);
-            }
-        }
+        odd_links(S.nbb, S.nb16, S.nb, S.nf, t, nb);
     }
 #pragma unroll
     for (int q = 0; q < 19; ++q) {

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "layout.h"
@@ -130,50 +131,25 @@ bool build_layout(const Dims& d, const std::vector<uint32_t>& fluid, int kper, i
             if (c < 0 || c >= out.ns) { err = "neighbour cell outside the layout"; return false; }
             out.nb[(size_t)(q - 1) * nf + t] = c;
         }
-    // short neighbour rows: per wave of 64 nodes and direction the base cells of its two
-    // classes of links (fluid cells, extra cells), per node a 15-bit offset and the class bit
+    // short neighbour rows, encoded over wave ranges on the host's threads
     {
-        const int nw = (nf + 63) / 64;
+        const int64_t nw = ((int64_t)nf + 63) / 64;
         out.nbb.assign((size_t)nw * NB_WAVE, 0);
         out.nb16.assign((size_t)9 * nf, 0u);
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<int> esc(nth, 0);
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < nth; ++k)
+            th.emplace_back([&, k] {
+                const int64_t a = nw * k / nth, b = nw * (k + 1) / nth;
+                esc[k] = encode_short_rows(out.nb.data(), nf, a, b, [](int64_t) { return true; }, out.nb16.data(),
+                                           out.nbb.data());
+            });
+        for (auto& x : th) x.join();
         out.n_escaped = 0;
-        for (int w = 0; w < nw; ++w) {
-            const int t0 = w * 64, t1 = std::min(nf, t0 + 64);
-            int32_t* B = &out.nbb[(size_t)w * NB_WAVE];
-            bool esc = false;
-            for (int q = 1; q < 19; ++q)
-                for (int c = 0; c < 2; ++c) {
-                    int32_t lo = INT32_MAX, hi = INT32_MIN;
-                    for (int t = t0; t < t1; ++t) {
-                        const int32_t v = out.nb[(size_t)(q - 1) * nf + t];
-                        if ((v >= nf) != (c == 1)) continue;
-                        lo = std::min(lo, v);
-                        hi = std::max(hi, v);
-                    }
-                    if (lo == INT32_MAX) lo = hi = 0;
-                    if ((int64_t)hi - lo >= 32768) esc = true;
-                    B[2 * (q - 1) + c] = lo;
-                }
-            if (esc) {
-                B[36] = 1;
-                ++out.n_escaped;
-                continue;
-            }
-            for (int q = 1; q < 19; ++q)
-                for (int t = t0; t < t1; ++t) {
-                    const int32_t v = out.nb[(size_t)(q - 1) * nf + t];
-                    const int c = v >= nf;
-                    const uint32_t u = (uint32_t)(v - B[2 * (q - 1) + c]) | (uint32_t)c << 15;
-                    out.nb16[(size_t)((q - 1) >> 1) * nf + t] |= u << (16 * ((q - 1) & 1));
-                }
-            // decoded exactly as k_collide decodes them
-            for (int q = 1; q < 19; ++q)
-                for (int t = t0; t < t1; ++t) {
-                    const uint32_t pr = out.nb16[(size_t)((q - 1) >> 1) * nf + t];
-                    const uint32_t u = ((q - 1) & 1) ? pr >> 16 : pr & 0xFFFFu;
-                    const int32_t cell = B[2 * (q - 1) + (int)(u >> 15)] + (int32_t)(u & 0x7FFFu);
-                    if (cell != out.nb[(size_t)(q - 1) * nf + t]) { err = "short neighbour row does not decode"; return false; }
-                }
+        for (int e : esc) {
+            if (e < 0) { err = "short neighbour row does not decode"; return false; }
+            out.n_escaped += e;
         }
     }
     // periodic pairs, last writer wins

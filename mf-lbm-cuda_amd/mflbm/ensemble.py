@@ -171,8 +171,8 @@ def main(argv=None):
         "steady": int((steps > 0).sum()), "steps_min": int(np.abs(steps).min()), "steps_mean": float(np.abs(steps).mean()),
         "steps_max": int(np.abs(steps).max()),
         "fluid_glups_device": node_steps / (out["device_ms"] * 1e-3) / 1e9,
-        # 19 populations read and written (4 B), plus the 72 B neighbour row on odd steps
-        "hbm_gbs_algorithmic": node_steps * (152.0 + 36.0) / (out["device_ms"] * 1e-3) / 1e9,
+        # 19 populations read and written (4 B), plus the 38.5 B short neighbour row on odd steps
+        "hbm_gbs_algorithmic": node_steps * (152.0 + 19.25) / (out["device_ms"] * 1e-3) / 1e9,
         "permeability_min": float(out["permeability"].min()), "permeability_max": float(out["permeability"].max()),
         "rho_ref_range": [float(ens.rho_ref.min()), float(ens.rho_ref.max())],
     }
