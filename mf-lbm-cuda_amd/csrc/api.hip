@@ -543,15 +543,16 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
                 }
         HIPCHK(h, upload_i32_raw(&h->tag, tag));
         h->split_gradient = (p.flags & MFLBM_FLAG_SPLIT_GRADIENT) != 0;
-        // Gradient chain.  The z-march (one pass, LDS rings) advances one plane per ~5 us
-        // whatever the tile area, so it pays where its z-runs are long; the two-pass 3-D
-        // tiles launch thousands of small workgroups and win where the runs are short, and
-        // in FP32, where their extra arithmetic (halo recomputation in three dimensions)
-        // costs less.  Measured (FP64 / FP32 ms per launch): 256^3 march 0.651 / 0.495,
-        // tiles 0.774 / 0.421; 64^3 march 0.143 / 0.105, tiles 0.026 / 0.021.  FP32 marches
-        // with 6 waves per workgroup (3 per SIMD, 5 % faster); FP64 with 4 (6 measured 60 %
-        // slower).  MFLBM_GRAD_SHAPE (0-4 march shapes, 5 tiles) and MFLBM_GRAD_KCHUNK
-        // (march z-run) override the choice (tuning).
+        // Gradient chain.  The two-pass 3-D tiles launch thousands of small workgroups; the
+        // z-march (one pass, LDS rings) advances one plane per ~5 us whatever the tile area,
+        // so it pays only where its z-runs are long, and loses where every node carries
+        // interface.  Measured (FP64 / FP32 ms per step of the chain): 256^3 drainage march
+        // 0.654 / 0.495, tiles 0.652 / 0.346; the same after 5 000 steps 0.660 / -, tiles
+        // 0.651 / 0.346; random colour field march 0.990, tiles 0.747 (FP64); 64^3 march
+        // 0.143 / 0.105, tiles 0.026 / 0.021.  The tiles are the default; MFLBM_GRAD_SHAPE
+        // (0-4 march shapes, 5 tiles) and MFLBM_GRAD_KCHUNK (march z-run) select the march
+        // (tuning).  FP32 marches with 6 waves per workgroup (3 per SIMD, 5 % faster); FP64
+        // with 4 (6 measured 60 % slower).
         const char* shape_env = std::getenv("MFLBM_GRAD_SHAPE");
         const char* kc_env = std::getenv("MFLBM_GRAD_KCHUNK");
         h->grad_shape = p.precision == MFLBM_FP32 ? GRAD_SHAPE_FP32 : GRAD_SHAPE_DEFAULT;
@@ -565,8 +566,7 @@ int mflbm_create(const mflbm_params* params, const int8_t* geometry, mflbm_handl
         const int tiles = h->tiles_x * h->tiles_y;
         int kc = std::max(16, (d.nz * tiles + 511) / 512);
         if (kc_env) kc = std::max(1, std::atoi(kc_env));
-        if (!shape_env && !kc_env && (p.precision == MFLBM_FP32 || kc < GRAD_MARCH_MIN_RUN))
-            h->grad_shape = GRAD_SHAPE_TILES;
+        if (!shape_env && !kc_env) h->grad_shape = GRAD_SHAPE_TILES;
         h->kchunk = std::min(kc, d.nz);
         // Two-pass chain: walk the boxes x fastest, then z, then y, so that the boxes above
         // and below one another run close together and share their z halo in L2; the y halo

@@ -178,8 +178,6 @@ template <>
 struct GradTiles<float> {
     static constexpr int N[3] = {16, 8, 4}, F[3] = {16, 8, 4};
 };
-// shortest z-run for which the FP64 march is chosen by default
-constexpr int GRAD_MARCH_MIN_RUN = 64;
 
 // Uniform deviate in [0, 1) of lattice node n under a seed, for the random initial phase
 // distribution: the splitmix64 output function (Steele, Lea and Flood, OOPSLA 2014) of
