@@ -170,6 +170,32 @@ def counter_pass(counters, geo_path, args, timeout=240):
     return vals
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VMEM_RD",
+               "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES")
+
+
+def sq_counters(args):
+    """Instruction mix of the step kernels: one rocprofv3 --pmc pass of eight SQ counters over
+    the counter child (steps 1-4 of the workload), printed as per-dispatch means and per-wave
+    figures."""
+    geo, _ = cases.drainage256(n=(args.n, args.n, args.n))
+    tmp = tempfile.mkdtemp(prefix="mflbm_geo_", dir="/tmp")
+    try:
+        path = os.path.join(tmp, "geometry.npy")
+        np.save(path, geo)
+        vals = counter_pass(list(SQ_COUNTERS), path, args)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {}
+    for g, d in vals.items():
+        m = {c: float(np.mean(v)) for c, v in d.items()}
+        w = max(1.0, m.get("SQ_WAVES", 1.0))
+        out[g] = {"per_dispatch": m,
+                  "per_wave": {c.replace("SQ_INSTS_", "").lower(): round(m[c] / w, 1)
+                               for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD") if c in m}}
+    print(json.dumps({"precision": args.precision, "lattice": [args.n] * 3, "sq": out}), flush=True)
+
+
 GROUPS = ("collide_odd", "collide_even", "gradient", "grad_normals", "grad_force", "probe", "probe_blocked")
 
 
@@ -274,6 +300,8 @@ def main():
                     help="measure HBM bytes per kernel with two rocprofv3 --pmc passes")
     ap.add_argument("--other-steps", type=int, default=1000,
                     help="timed steps of the same workload in the other precision (0 = skip)")
+    ap.add_argument("--sq-counters", action="store_true",
+                    help="print the step kernels' SQ instruction counts (one rocprofv3 --pmc pass) and exit")
     ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -282,6 +310,9 @@ def main():
         return
     if args.cpu_child:
         cpu_baseline_child(args)
+        return
+    if args.sq_counters:
+        sq_counters(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU z-slab "
